@@ -1,0 +1,166 @@
+"""CPU oracle for the direct-SCF G[D] path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import
+this package, as the checker and the CPU baseline.  The product package
+(``chronusq_public_amd``) never imports it.
+
+Python bindings (ctypes) over ``liboracle.so`` built from ``cq_oracle.cpp`` plus a numpy
+restatement of the RHF SCF driver (``scf.py``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_SRC = os.path.join(_HERE, "cq_oracle.cpp")
+
+COULOMB, EXCHANGE = 0, 1
+
+
+def build(force: bool = True) -> str:
+    """Compile liboracle.so with g++ (portable x86-64-v3 target, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        cmd = ["g++", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
+               "-o", _LIB_PATH + ".tmp", _SRC]
+        subprocess.run(cmd, check=True)
+        os.replace(_LIB_PATH + ".tmp", _LIB_PATH)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build(force=True)
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER
+        dp = P(ctypes.c_double)
+        ip = P(ctypes.c_int)
+        L.orc_basis_create.restype = ctypes.c_void_p
+        L.orc_basis_create.argtypes = [ctypes.c_int, ip, ip, ip, dp, dp, dp]
+        L.orc_basis_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_nbf.restype = ctypes.c_int
+        L.orc_nbf.argtypes = [ctypes.c_void_p]
+        L.orc_boys.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+        L.orc_quartet.restype = ctypes.c_int
+        L.orc_quartet.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [dp]
+        L.orc_car2sph.argtypes = [ctypes.c_int, dp]
+        L.orc_schwarz.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
+        L.orc_shblk_norm.argtypes = [ctypes.c_void_p, dp, dp]
+        L.orc_contract_direct.restype = ctypes.c_long
+        L.orc_contract_direct.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_int,
+                                          P(dp), P(dp), ip, dp, ctypes.c_int]
+        L.orc_eri_tensor.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
+        L.orc_one_electron.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def boys(mmax: int, T: float) -> np.ndarray:
+    out = np.zeros(mmax + 1)
+    lib().orc_boys(mmax, float(T), _dp(out))
+    return out
+
+
+def car2sph(l: int) -> np.ndarray:
+    nc = (l + 1) * (l + 2) // 2
+    out = np.zeros((2 * l + 1, nc))
+    lib().orc_car2sph(l, _dp(out))
+    return out
+
+
+class OracleBasis:
+    """Oracle view of a ``chronusq_public_amd.basis.BasisSet`` (flat shell arrays)."""
+
+    def __init__(self, basis):
+        l, pure, nprim, alpha, coef, origin = basis.flat()
+        self._keep = (np.ascontiguousarray(l, dtype=np.int32), np.ascontiguousarray(pure, dtype=np.int32),
+                      np.ascontiguousarray(nprim, dtype=np.int32), np.ascontiguousarray(alpha),
+                      np.ascontiguousarray(coef), np.ascontiguousarray(origin))
+        k = self._keep
+        self.h = lib().orc_basis_create(len(k[0]), _ip(k[0]), _ip(k[1]), _ip(k[2]), _dp(k[3]), _dp(k[4]), _dp(k[5]))
+        self.basis = basis
+        self.nshell = len(k[0])
+        self.nbf = lib().orc_nbf(self.h)
+        self.sizes = [s.size for s in basis.shells]
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_basis_destroy(self.h)
+        except Exception:
+            pass
+
+    def quartet(self, s1, s2, s3, s4, algo: int = 0) -> np.ndarray:
+        n = [self.sizes[s] for s in (s1, s2, s3, s4)]
+        out = np.zeros(int(np.prod(n)))
+        lib().orc_quartet(self.h, s1, s2, s3, s4, algo, _dp(out))
+        return out.reshape(n)
+
+    def schwarz(self, nthreads: int = 8) -> np.ndarray:
+        Q = np.zeros((self.nshell, self.nshell))
+        lib().orc_schwarz(self.h, _dp(Q), nthreads)
+        return Q
+
+    def shblk_norm(self, X: np.ndarray) -> np.ndarray:
+        Xf = np.asfortranarray(X, dtype=np.float64)
+        out = np.zeros((self.nshell, self.nshell), order="F")
+        lib().orc_shblk_norm(self.h, _dp(Xf), _dp(out))
+        return out
+
+    def contract_direct(self, Q: np.ndarray, tau: float, X: Sequence[np.ndarray],
+                        AX: Sequence[np.ndarray], types: Sequence[int], scales: Sequence[float],
+                        nthreads: int = 8) -> int:
+        """newTwoEContractDirect restatement; AX (Fortran-ordered float64) accumulate in place."""
+        n = len(X)
+        Xf = [np.asfortranarray(x, dtype=np.float64) for x in X]
+        for a in AX:
+            if not (a.flags.f_contiguous and a.dtype == np.float64):
+                raise ValueError("AX must be Fortran-ordered float64 arrays")
+        Qf = np.asfortranarray(Q, dtype=np.float64)
+        xp = (ctypes.POINTER(ctypes.c_double) * n)(*[_dp(x) for x in Xf])
+        ap = (ctypes.POINTER(ctypes.c_double) * n)(*[_dp(a) for a in AX])
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        s = np.ascontiguousarray(scales, dtype=np.float64)
+        return int(lib().orc_contract_direct(self.h, _dp(Qf), float(tau), n, xp, ap, _ip(t), _dp(s), nthreads))
+
+    def eri_tensor(self, nthreads: int = 8) -> np.ndarray:
+        N = self.nbf
+        out = np.zeros((N, N, N, N))
+        lib().orc_eri_tensor(self.h, _dp(out), nthreads)
+        return out
+
+    def one_electron(self, mol):
+        N = self.nbf
+        S = np.zeros((N, N), order="F")
+        T = np.zeros((N, N), order="F")
+        V = np.zeros((N, N), order="F")
+        Z = np.ascontiguousarray(mol.Z, dtype=np.float64)
+        xyz = np.ascontiguousarray(mol.coords_bohr, dtype=np.float64)
+        lib().orc_one_electron(self.h, mol.natom, _dp(Z), _dp(xyz), _dp(S), _dp(T), _dp(V))
+        return S, T, V
+
+
+def brute_force_contract(eri: np.ndarray, X: np.ndarray, ctype: int, scale: float) -> np.ndarray:
+    """In-core formulas of newTwoEContractIncore (aointegrals_contract.h:239-259):
+    COULOMB: 2*scale*sum_{kl} (ij|kl) X_lk ; EXCHANGE: 2*scale*sum_{kl} (il|kj) X_lk."""
+    if ctype == COULOMB:
+        return 2.0 * scale * np.einsum("ijkl,lk->ij", eri, X)
+    return 2.0 * scale * np.einsum("ilkj,lk->ij", eri, X)
