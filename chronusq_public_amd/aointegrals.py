@@ -1,0 +1,175 @@
+"""Host-side mirror of ChronusQ's ``AOIntegrals`` for the direct G[D] path.
+
+Same names and argument meaning as the reference operator interface
+(include/aointegrals.h:468-502):
+
+* ``COULOMB``, ``EXCHANGE`` (``ERI_CONTRACTION_TYPE``, include/aointegrals.h:468-472);
+* ``newTwoEContract(X, AX, contractionList, scalingFactors)`` dispatches on
+  ``integralAlgorithm`` (include/aointegrals.h:489-502) -- only ``DIRECT`` exists here; any
+  other value raises, as the reference's ``CErr`` does;
+* ``newTwoEContractDirect`` accumulates ``AX[i] += 2*scale*J[X]`` or ``2*scale*K[X]``
+  (include/aointegrals/aointegrals_contract.h:24-194);
+* ``computeSchwartz`` / ``schwartz`` (src/aointegrals/aointegrals_builders.cpp:469-518);
+* ``thresholdSchwartz`` (include/aointegrals.h:334, default 1e-14).
+
+Matrices may be numpy arrays (host; column-major is used as in Eigen, C-ordered arrays are
+accepted as their transposes, which leaves J and K of any density consistent) or torch tensors
+on the GPU (device-resident path, no host copies).  All arithmetic runs in libcqgpu.so on
+the GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import CQ_COULOMB as COULOMB, CQ_EXCHANGE as EXCHANGE, check
+
+DIRECT = "DIRECT"
+INCORE = "INCORE"
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class AOIntegrals:
+    def __init__(self, basis, integral_algorithm: str = DIRECT):
+        self.basis = basis
+        self.integralAlgorithm = integral_algorithm
+        L = _lib.load()
+        l, pure, nprim, alpha, coef, origin = basis.flat()
+        self._flat = (np.ascontiguousarray(l, np.int32), np.ascontiguousarray(pure, np.int32),
+                      np.ascontiguousarray(nprim, np.int32), np.ascontiguousarray(alpha, np.float64),
+                      np.ascontiguousarray(coef, np.float64), np.ascontiguousarray(origin, np.float64))
+        f = self._flat
+        h = ctypes.c_void_p()
+        rc = L.cq_ctx_create(ctypes.byref(h), len(f[0]), _ip(f[0]), _ip(f[1]), _ip(f[2]), _dp(f[3]),
+                             _dp(f[4]), _dp(f[5]))
+        check(rc, None)
+        self._h = h
+        dims = [ctypes.c_int() for _ in range(5)]
+        check(L.cq_get_dims(self._h, *[ctypes.byref(d) for d in dims]), self._h)
+        self.nShell, self.nBasis, self.nCart, self.nShellPair, self.nPrimPair = (d.value for d in dims)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().cq_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ screening
+    @property
+    def thresholdSchwartz(self) -> float:
+        v = ctypes.c_double()
+        check(_lib.load().cq_get_schwarz_threshold(self._h, ctypes.byref(v)), self._h)
+        return v.value
+
+    @thresholdSchwartz.setter
+    def thresholdSchwartz(self, tau: float):
+        check(_lib.load().cq_set_schwarz_threshold(self._h, float(tau)), self._h)
+
+    def computeSchwartz(self) -> np.ndarray:
+        Q = np.zeros((self.nShell, self.nShell))
+        check(_lib.load().cq_get_schwarz(self._h, _dp(Q)), self._h)
+        return Q
+
+    @property
+    def schwartz(self) -> np.ndarray:
+        return self.computeSchwartz()
+
+    # ------------------------------------------------------------------ contraction
+    def newTwoEContract(self, X: Sequence, AX: Sequence, contractionList: Sequence[int],
+                        scalingFactors: Sequence[float]) -> None:
+        if self.integralAlgorithm == DIRECT:
+            self.newTwoEContractDirect(X, AX, contractionList, scalingFactors)
+        else:
+            raise _lib.CQError(f"Integral algorithm {self.integralAlgorithm} is not available "
+                               "(only DIRECT is implemented on the GPU)")
+
+    def newTwoEContractDirect(self, X: Sequence, AX: Sequence, contractionList: Sequence[int],
+                              scalingFactors: Sequence[float], stream=None) -> None:
+        n = len(X)
+        if not (len(AX) == n == len(contractionList) == len(scalingFactors)):
+            raise ValueError("X, AX, contractionList and scalingFactors must have equal length")
+        types = np.ascontiguousarray(contractionList, dtype=np.int32)
+        scales = np.ascontiguousarray(scalingFactors, dtype=np.float64)
+        L = _lib.load()
+        if _is_torch_cuda(X[0]):
+            import torch
+            for t in list(X) + list(AX):
+                if not (_is_torch_cuda(t) and t.dtype == torch.float64 and t.is_contiguous()):
+                    raise TypeError("device matrices must be contiguous float64 CUDA tensors")
+                if t.shape != (self.nBasis, self.nBasis):
+                    raise ValueError("matrix shape does not match the basis")
+            if stream is None:
+                stream = torch.cuda.current_stream(X[0].device).cuda_stream
+            xp = (ctypes.c_void_p * n)(*[t.data_ptr() for t in X])
+            ap = (ctypes.c_void_p * n)(*[t.data_ptr() for t in AX])
+            rc = L.cq_contract_direct_device(self._h, n, xp, ap, _ip(types), _dp(scales), self.nBasis,
+                                             self.nBasis, ctypes.c_void_p(stream))
+            check(rc, self._h)
+            return
+        xs: List[np.ndarray] = []
+        for x in X:
+            x = np.asarray(x)
+            if x.dtype != np.float64 or x.shape != (self.nBasis, self.nBasis):
+                raise ValueError("matrices must be float64 and match the basis dimension")
+            xs.append(np.asfortranarray(x) if not x.flags.c_contiguous else x)
+        # C-ordered arrays are passed as their transposes (see module docstring)
+        for a in AX:
+            if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not (a.flags.f_contiguous or a.flags.c_contiguous):
+                raise ValueError("AX must be contiguous float64 numpy arrays")
+        xp = (ctypes.POINTER(ctypes.c_double) * n)(*[_dp(x) for x in xs])
+        ap = (ctypes.POINTER(ctypes.c_double) * n)(*[_dp(a) for a in AX])
+        rc = L.cq_contract_direct(self._h, n, xp, ap, _ip(types), _dp(scales), self.nBasis, self.nBasis)
+        check(rc, self._h)
+
+    # ------------------------------------------------------------------ diagnostics
+    def quartet_counts(self) -> dict:
+        L = _lib.load()
+        nc = L.cq_num_class_pairs()
+        buf = (ctypes.c_longlong * nc)()
+        nout = ctypes.c_int()
+        check(L.cq_get_quartet_counts(self._h, buf, nc, ctypes.byref(nout)), self._h)
+        out = {}
+        for k in range(nc):
+            ls = [ctypes.c_int() for _ in range(4)]
+            L.cq_class_pair_info(k, *[ctypes.byref(x) for x in ls])
+            name = "({}{}|{}{})".format(*["spdfg"[x.value] for x in ls])
+            out[name] = int(buf[k])
+        return out
+
+    def quartets(self, shells4: np.ndarray) -> List[np.ndarray]:
+        """Contracted quartets (s1 s2|s3 s4) in the shells' own basis (as libint returns them)."""
+        s4 = np.ascontiguousarray(np.asarray(shells4, dtype=np.int32).reshape(-1, 4))
+        sizes = [s.size for s in self.basis.shells]
+        dims = [[sizes[i] for i in row] for row in s4]
+        total = int(sum(np.prod(d) for d in dims))
+        out = np.zeros(total)
+        check(_lib.load().cq_debug_quartets(self._h, len(s4), _ip(s4), _dp(out)), self._h)
+        res, off = [], 0
+        for d in dims:
+            n = int(np.prod(d))
+            res.append(out[off:off + n].reshape(d))
+            off += n
+        return res
+
+
+def _is_torch_cuda(x) -> bool:
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda

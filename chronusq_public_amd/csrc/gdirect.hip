@@ -1,0 +1,988 @@
+// MI355X (gfx950) direct-SCF two-electron contraction G[D].
+//
+// Drop-in replacement for ChronusQ's AOIntegrals::newTwoEContractDirect<Op,T>
+// (include/aointegrals/aointegrals_contract.h:24-194, declared include/aointegrals.h:474-480),
+// exposed through the C ABI declared in include/cqgpu.h.
+//
+// Semantics reproduced (per input matrix X_i with type t_i and scale s_i):
+//   COULOMB : AX_i += 2 s_i J[X_i],  J_{mu nu} = sum_{lam sig} (mu nu|lam sig) X_{sig lam}
+//   EXCHANGE: AX_i += 2 s_i K[X_i],  K_{mu nu} = sum_{lam sig} (mu lam|nu sig) X_{lam sig}
+// which is the reference's  AX += 1/2 s G  with its 4 Coulomb / 8 exchange scatter updates
+// per integral (aointegrals_contract.h:145-193).  Quartets are screened exactly as the
+// reference does (aointegrals_contract.h:111-131): skip when
+//   max(N(s1,s4),N(s2,s4),N(s3,s4),N(s1,s3),N(s2,s3),N(s1,s2)) * Q(s1,s2) * Q(s3,s4) < tau
+// evaluated left to right in FP64 on the canonical (s1 >= s2, (s3,s4) <= (s1,s2)) order,
+// with N the block infinity norm maximised over all input matrices.
+//
+// Layout in HBM (all device-resident for the life of a context):
+//   * shell pairs sorted by class (la lb), la >= lb, then by Schwarz bound descending;
+//   * primitive pairs (PrimPair, 96 B AoS) referenced by pair ranges;
+//   * Boys table F_m(T_i) (361 x 25 doubles);
+//   * per call: Cartesian densities (row-major, ncart^2), forward J/K accumulators,
+//     (plus transposed-density accumulators used only for a non-symmetric X),
+//     compacted quartet task lists per class pair (uint2 {bra pair, ket pair}).
+//
+// The whole G[D] runs in the Cartesian AO basis: D_cart = T^T X T once per call, all
+// quartets digest Cartesian integrals, G_sph = T G_cart T^T once at the end.  T is the
+// block-diagonal real-solid-harmonic transform for pure shells (identity otherwise).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cqgpu.h"
+
+#include "gdirect_kernels.h"
+#include "_gen/eri_registry.inc"
+
+static ClassKernels kClassKernels[CQ_NUM_CLASS_PAIRS];
+static const int kNumClassPairs = CQ_NUM_CLASS_PAIRS;
+static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
+
+// ---------------------------------------------------------------------------------------
+// Screening: one wavefront per (class pair, bra pair); compacted task lists.
+// ---------------------------------------------------------------------------------------
+struct CPMeta {
+  int bra_start, nbra, ket_start, nket, same, pad;
+  long long task_off;
+};
+
+__global__ void __launch_bounds__(WG) screen_kernel(
+    const CPMeta* __restrict__ cps, int ncp, const int* __restrict__ item_start, int nitem,
+    const PairInfo* __restrict__ pairs, const double* __restrict__ pairQ,
+    const double* __restrict__ Nmax, const unsigned long long* __restrict__ nglob_bits, int nshell,
+    double tau, uint2* __restrict__ tasks, unsigned long long* __restrict__ counts) {
+  const int wave = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wave >= nitem) return;
+  int cp = 0;
+  while (cp + 1 < ncp && item_start[cp + 1] <= wave) ++cp;
+  const CPMeta m = cps[cp];
+  const int bra_local = wave - item_start[cp];
+  const int P = m.bra_start + bra_local;
+  const int ket_end = m.same ? bra_local + 1 : m.nket;
+  const double Qp = pairQ[P];
+  const PairInfo pb = pairs[P];
+  const long canonP = (long)pb.s1 * (pb.s1 + 1) / 2 + pb.s2;
+  const double nglob = __longlong_as_double((long long)*nglob_bits);
+  for (int j0 = 0; j0 < ket_end; j0 += 64) {
+    const double Qj0 = pairQ[m.ket_start + j0];
+    if (nglob * Qp * Qj0 * (1.0 + 1e-12) < tau) break;
+    const int j = j0 + lane;
+    bool pass = false;
+    if (j < ket_end) {
+      const int Kidx = m.ket_start + j;
+      const double Qk = pairQ[Kidx];
+      const PairInfo pk = pairs[Kidx];
+      const long canonK = (long)pk.s1 * (pk.s1 + 1) / 2 + pk.s2;
+      int s1, s2, s3, s4;
+      double Q12, Q34;
+      if (canonP >= canonK) { s1 = pb.s1; s2 = pb.s2; s3 = pk.s1; s4 = pk.s2; Q12 = Qp; Q34 = Qk; }
+      else                  { s1 = pk.s1; s2 = pk.s2; s3 = pb.s1; s4 = pb.s2; Q12 = Qk; Q34 = Qp; }
+      const double n14 = Nmax[s1 * nshell + s4], n24 = Nmax[s2 * nshell + s4];
+      const double n34 = Nmax[s3 * nshell + s4], n13 = Nmax[s1 * nshell + s3];
+      const double n23 = Nmax[s2 * nshell + s3], n12 = Nmax[s1 * nshell + s2];
+      const double mx = fmax(n14, fmax(n24, fmax(n34, fmax(n13, fmax(n23, n12)))));
+      const double bound = __dmul_rn(__dmul_rn(mx, Q12), Q34);
+      pass = !(bound < tau);
+    }
+    const unsigned long long mask = __ballot(pass);
+    if (mask) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&counts[cp], (unsigned long long)__popcll(mask));
+      base = __shfl(base, 0);
+      if (pass) {
+        const unsigned long long below = mask & ((1ull << lane) - 1ull);
+        tasks[m.task_off + base + __popcll(below)] = make_uint2((unsigned)P, (unsigned)(m.ket_start + j));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Shell metadata and the solid-harmonic transform on the device
+// ---------------------------------------------------------------------------------------
+struct ShellMeta {
+  int l, pure, sph_off, cart_off;
+};
+
+// Coefficient T(mu, a) between spherical function mu (index within shell) and Cartesian
+// component a (index within shell) of a shell.
+__device__ __forceinline__ double tcoef(const ShellMeta& s, int mu, int a, const double* __restrict__ tco,
+                                        const int* __restrict__ tco_off) {
+  if (!s.pure) return mu == a ? 1.0 : 0.0;
+  const int nc = (s.l + 1) * (s.l + 2) / 2;
+  return tco[tco_off[s.l] + mu * nc + a];
+}
+
+#define MAX_MATS 8
+struct MatList {
+  const double* X[MAX_MATS];
+  int n;
+};
+
+// Block infinity norms of every (s,t) shell block, maximised over the matrices; also the
+// global maximum (as the bit pattern of a non-negative double, so atomicMax orders it).
+__global__ void norm_kernel(MatList ml, int ld, const ShellMeta* __restrict__ sh, int nshell,
+                            double* __restrict__ Nmax, unsigned long long* __restrict__ nglob_bits) {
+  const int st = blockIdx.x * blockDim.x + threadIdx.x;
+  if (st >= nshell * nshell) return;
+  const int s = st / nshell, t = st - s * nshell;
+  const ShellMeta A = sh[s], B = sh[t];
+  const int na = A.pure ? 2 * A.l + 1 : (A.l + 1) * (A.l + 2) / 2;
+  const int nb = B.pure ? 2 * B.l + 1 : (B.l + 1) * (B.l + 2) / 2;
+  double mx = 0.0;
+  for (int m = 0; m < ml.n; ++m) {
+    const double* X = ml.X[m];
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i < na; ++i) mx = fmax(mx, fabs(X[(long)(B.sph_off + j) * ld + A.sph_off + i]));
+  }
+  Nmax[st] = mx;
+  atomicMax(nglob_bits, (unsigned long long)__double_as_longlong(mx));
+}
+
+// flag = 1 when X is not exactly symmetric.
+__global__ void nonsym_kernel(const double* __restrict__ X, int n, int ld, int* flag) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)n * n) return;
+  const int i = k % n, j = k / n;
+  if (i < j && X[(long)j * ld + i] != X[(long)i * ld + j]) atomicOr(flag, 1);
+}
+
+// Cartesian density (row-major ncart x ncart): Dc = T^T X T.
+__global__ void cart_density_kernel(const double* __restrict__ X, int ld, const ShellMeta* __restrict__ sh,
+                                    const int* __restrict__ cart2shell, int ncart,
+                                    const double* __restrict__ tco, const int* __restrict__ tco_off,
+                                    double* __restrict__ Dc) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)ncart * ncart) return;
+  const int a = k / ncart, b = k - (long)a * ncart;
+  const int sa = cart2shell[a], sb = cart2shell[b];
+  const ShellMeta A = sh[sa], B = sh[sb];
+  const int ia = a - A.cart_off, ib = b - B.cart_off;
+  const int na = A.pure ? 2 * A.l + 1 : (A.l + 1) * (A.l + 2) / 2;
+  const int nb = B.pure ? 2 * B.l + 1 : (B.l + 1) * (B.l + 2) / 2;
+  double sk = 0.0;
+  for (int mu = 0; mu < na; ++mu) {
+    const double ta = tcoef(A, mu, ia, tco, tco_off);
+    if (ta == 0.0) continue;
+    for (int nu = 0; nu < nb; ++nu) {
+      const double tb = tcoef(B, nu, ib, tco, tco_off);
+      if (tb == 0.0) continue;
+      sk += ta * tb * X[(long)(B.sph_off + nu) * ld + A.sph_off + mu];
+    }
+  }
+  Dc[k] = sk;
+}
+
+// AX[mu + nu*ld] += f * sum_{a,c} T(mu,a) T(nu,c) (G1[a][c] + G2[c][a])
+__global__ void finalize_kernel(double* __restrict__ AX, int n, int ld, double f,
+                                const double* __restrict__ G1, const double* __restrict__ G2,
+                                const int* __restrict__ nonsym, const double* __restrict__ G2alt,
+                                const ShellMeta* __restrict__ sh, const int* __restrict__ sph2shell, int ncart,
+                                const double* __restrict__ tco, const int* __restrict__ tco_off) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)n * n) return;
+  const int mu = k % n, nu = k / n;
+  const double* g2 = (nonsym && *nonsym) ? G2alt : G2;
+  const ShellMeta A = sh[sph2shell[mu]], B = sh[sph2shell[nu]];
+  const int im = mu - A.sph_off, in = nu - B.sph_off;
+  const int nca = (A.l + 1) * (A.l + 2) / 2, ncb = (B.l + 1) * (B.l + 2) / 2;
+  double s = 0.0;
+  for (int a = 0; a < nca; ++a) {
+    const double ta = tcoef(A, im, a, tco, tco_off);
+    if (ta == 0.0) continue;
+    for (int c = 0; c < ncb; ++c) {
+      const double tc = tcoef(B, in, c, tco, tco_off);
+      if (tc == 0.0) continue;
+      const long ia = A.cart_off + a, ic = B.cart_off + c;
+      s += ta * tc * (G1[ia * ncart + ic] + g2[ic * ncart + ia]);
+    }
+  }
+  AX[(long)nu * ld + mu] += f * s;
+}
+
+// Schwarz: Q(P) = sqrt(max_{ij} |(ij|ij)|) in the shells' own (pure or Cartesian) basis,
+// from the Cartesian diagonal block (ab|ab) of each pair.
+__global__ void schwarz_kernel(const double* __restrict__ blocks, const int* __restrict__ pair_ids, int npair,
+                               const PairInfo* __restrict__ pairs, const ShellMeta* __restrict__ sh,
+                               const double* __restrict__ tco, const int* __restrict__ tco_off,
+                               double* __restrict__ pairQ) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npair) return;
+  const int P = pair_ids[k];
+  const PairInfo pi = pairs[P];
+  const ShellMeta A = sh[pi.a], B = sh[pi.b];
+  const int nca = (A.l + 1) * (A.l + 2) / 2, ncb = (B.l + 1) * (B.l + 2) / 2;
+  const int nab = nca * ncb;
+  const int na = A.pure ? 2 * A.l + 1 : nca;
+  const int nb = B.pure ? 2 * B.l + 1 : ncb;
+  const double* blk = blocks + (long)k * nab * nab;
+  double mx = 0.0;
+  for (int i = 0; i < na; ++i)
+    for (int j = 0; j < nb; ++j) {
+      double s = 0.0;
+      for (int a = 0; a < nca; ++a) {
+        const double ta = tcoef(A, i, a, tco, tco_off);
+        if (ta == 0.0) continue;
+        for (int b = 0; b < ncb; ++b) {
+          const double tb = tcoef(B, j, b, tco, tco_off);
+          if (tb == 0.0) continue;
+          for (int c = 0; c < nca; ++c) {
+            const double tc = tcoef(A, i, c, tco, tco_off);
+            if (tc == 0.0) continue;
+            for (int d = 0; d < ncb; ++d) {
+              const double td = tcoef(B, j, d, tco, tco_off);
+              if (td == 0.0) continue;
+              s += ta * tb * tc * td * blk[(long)(a * ncb + b) * nab + c * ncb + d];
+            }
+          }
+        }
+      }
+      mx = fmax(mx, fabs(s));
+    }
+  pairQ[P] = sqrt(mx);
+}
+
+// =======================================================================================
+// Host side
+// =======================================================================================
+namespace {
+
+int ncart_l(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Real solid harmonics in the uniformly normalised Cartesian basis of a shell
+// (every Cartesian component carries the x^l normalisation), m = -l..l, unit norm,
+// Helgaker-Jorgensen-Olsen expansion (Molecular Electronic-Structure Theory, eq. 6.4.47-52).
+std::vector<double> solid_harmonics(int l) {
+  const int nc = ncart_l(l);
+  std::vector<int> cx, cy, cz;
+  for (int i = 0; i <= l; ++i)
+    for (int j = 0; j <= i; ++j) { cx.push_back(l - i); cy.push_back(i - j); cz.push_back(j); }
+  auto fac = [](int n) { double r = 1; for (int k = 2; k <= n; ++k) r *= k; return r; };
+  auto bin = [&](int n, int k) { return (k < 0 || k > n) ? 0.0 : fac(n) / (fac(k) * fac(n - k)); };
+  auto dfac = [](int n) { double r = 1; while (n > 1) { r *= n; n -= 2; } return r; };
+  auto comp = [&](int x, int y, int z) {
+    for (int k = 0; k < nc; ++k) if (cx[k] == x && cy[k] == y && cz[k] == z) return k;
+    return -1;
+  };
+  std::vector<double> M((size_t)(2 * l + 1) * nc, 0.0);
+  for (int m = -l; m <= l; ++m) {
+    const int am = std::abs(m);
+    const int twovm = m < 0 ? 1 : 0;   // 2 * v_m
+    std::vector<double> c(nc, 0.0);
+    for (int t = 0; t <= (l - am) / 2; ++t)
+      for (int u = 0; u <= t; ++u)
+        for (int twov = twovm; twov <= am; twov += 2) {
+          // v = twov/2 ; sign (-1)^(t + v - v_m)
+          const int sgn_exp = t + (twov - twovm) / 2;
+          const double C = ((sgn_exp & 1) ? -1.0 : 1.0) * std::pow(0.25, t) * bin(l, t) * bin(l - t, am + t) *
+                           bin(t, u) * bin(am, twov);
+          const int ex = 2 * t + am - 2 * u - twov, ey = 2 * u + twov, ez = l - 2 * t - am;
+          if (ex < 0) continue;
+          const int k = comp(ex, ey, ez);
+          if (k >= 0) c[k] += C;
+        }
+    // normalise in the metric <g_n|g_n'> = prod_d (n_d + n'_d - 1)!! / (2l-1)!!
+    double nrm = 0.0;
+    for (int a = 0; a < nc; ++a)
+      for (int b = 0; b < nc; ++b) {
+        const int sx = cx[a] + cx[b], sy = cy[a] + cy[b], sz = cz[a] + cz[b];
+        if ((sx | sy | sz) & 1) continue;
+        nrm += c[a] * c[b] * dfac(sx - 1) * dfac(sy - 1) * dfac(sz - 1) / dfac(2 * l - 1);
+      }
+    const double sc = 1.0 / std::sqrt(nrm);
+    for (int a = 0; a < nc; ++a) M[(size_t)(m + l) * nc + a] = c[a] * sc;
+  }
+  return M;
+}
+
+// Boys table F_m(i * step), m < BOYS_MTAB, plus exp(-T) (long-double series + downward
+// recursion).
+std::vector<double> boys_table() {
+  std::vector<double> tab((size_t)BOYS_NGRID * (BOYS_MTAB + 1));
+  for (int i = 0; i < BOYS_NGRID; ++i) {
+    const long double T = (long double)i * (long double)BOYS_STEP;
+    long double F[BOYS_MTAB];
+    const int mm = BOYS_MTAB - 1;
+    const long double eT = std::exp(-T);
+    if (T == 0.0L) {
+      for (int m = 0; m < BOYS_MTAB; ++m) F[m] = 1.0L / (2 * m + 1);
+    } else {
+      long double term = 1.0L / (2 * mm + 1), sum = term;
+      for (int k = 1; k < 4000; ++k) {
+        term *= 2.0L * T / (2 * mm + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-21L) break;
+      }
+      F[mm] = eT * sum;
+      for (int m = mm - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + eT) / (2 * m + 1);
+    }
+    for (int m = 0; m < BOYS_MTAB; ++m) tab[(size_t)i * (BOYS_MTAB + 1) + m] = (double)F[m];
+    tab[(size_t)i * (BOYS_MTAB + 1) + BOYS_MTAB] = (double)eT;
+  }
+  return tab;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    }                                                                                       \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t need) {
+    if (need <= n && p) return;
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    n = 0;
+    if (need == 0) return;
+    HIPCHK(hipMalloc(&p, need * sizeof(T)));
+    n = need;
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    ensure(v.size());
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct cq_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  int nshell = 0, nsph = 0, ncart = 0, maxl = 0, npc = 0;
+  std::vector<ShellMeta> shells;
+  std::vector<int> cart2shell, sph2shell;
+  std::vector<PairInfo> pairs;
+  std::vector<double> pairQ;
+  std::vector<int> class_start;      // npc + 1
+  std::vector<PrimPair> prims;
+  std::vector<double> Q;             // nshell x nshell
+  double tau = 1e-14;                // include/aointegrals.h:334
+  int precision_mode = 0;
+  double prim_threshold = 1e-17;
+  std::vector<CPMeta> cps;
+  std::vector<int> item_start;
+  int nitem = 0;
+  long long task_capacity = 0;
+  std::vector<unsigned long long> last_counts;
+  std::vector<double> tco;
+  std::vector<int> tco_off;
+  // device
+  DevBuf<ShellMeta> d_shells;
+  DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start, d_flags;
+  DevBuf<double> d_tco, d_boys, d_pairQ, d_Nmax, d_work, d_mats;
+  DevBuf<PairInfo> d_pairs;
+  DevBuf<PrimPair> d_prims;
+  DevBuf<CPMeta> d_cps;
+  DevBuf<uint2> d_tasks;
+  DevBuf<unsigned long long> d_counts, d_nglob;
+  std::string err;
+
+  ~cq_ctx() {
+    d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
+    d_item_start.release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
+    d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
+    d_cps.release(); d_tasks.release(); d_counts.release(); d_nglob.release();
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+
+int grid_for(long n, int block) { return (int)std::max<long>(1, (n + block - 1) / block); }
+
+int persistent_grid() {
+  static int g = 0;
+  if (!g) {
+    int dev = 0, ncu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    g = std::max(1, ncu) * 8;
+  }
+  return g;
+}
+
+void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                   const double* coef, const double* origin) {
+  HIPCHK(hipGetDevice(&c->device));
+  HIPCHK(hipStreamCreate(&c->stream));
+  c->nshell = nshell;
+  c->shells.resize(nshell);
+  std::vector<int> poff(nshell + 1, 0);
+  int sph = 0, cart = 0;
+  for (int s = 0; s < nshell; ++s) {
+    if (l[s] < 0 || l[s] > 2) throw std::invalid_argument("shell angular momentum above d is not supported");
+    if (nprim[s] < 1) throw std::invalid_argument("shell with no primitives");
+    c->shells[s] = ShellMeta{l[s], pure[s] ? 1 : 0, sph, cart};
+    const int nsz = pure[s] ? 2 * l[s] + 1 : ncart_l(l[s]);
+    for (int k = 0; k < nsz; ++k) c->sph2shell.push_back(s);
+    for (int k = 0; k < ncart_l(l[s]); ++k) c->cart2shell.push_back(s);
+    sph += nsz;
+    cart += ncart_l(l[s]);
+    poff[s + 1] = poff[s] + nprim[s];
+    c->maxl = std::max(c->maxl, l[s]);
+  }
+  c->nsph = sph;
+  c->ncart = cart;
+  c->npc = 6;   // ss ps pp ds dp dd
+  // solid-harmonic tables for l = 0..2
+  c->tco_off.assign(5, 0);
+  for (int L = 0; L <= 4; ++L) {
+    c->tco_off[L] = (int)c->tco.size();
+    if (L <= 2) {
+      auto M = solid_harmonics(L);
+      c->tco.insert(c->tco.end(), M.begin(), M.end());
+    }
+  }
+  // shell pairs (canonical s1 >= s2) and their primitive pairs
+  struct HostPair { PairInfo info; int cls; std::vector<PrimPair> prim; };
+  std::vector<HostPair> hp;
+  hp.reserve((size_t)nshell * (nshell + 1) / 2);
+  const double pi = 3.14159265358979323846;
+  for (int s1 = 0; s1 < nshell; ++s1)
+    for (int s2 = 0; s2 <= s1; ++s2) {
+      int a = s1, b = s2;
+      if (l[s2] > l[s1]) std::swap(a, b);
+      HostPair h;
+      h.info.a = a; h.info.b = b; h.info.s1 = s1; h.info.s2 = s2;
+      h.info.a0 = c->shells[a].cart_off; h.info.b0 = c->shells[b].cart_off;
+      h.cls = pair_class(l[a], l[b]);
+      const double* A = origin + 3 * a;
+      const double* B = origin + 3 * b;
+      const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+      for (int i = 0; i < nprim[a]; ++i)
+        for (int j = 0; j < nprim[b]; ++j) {
+          const double ea = alpha[poff[a] + i], eb = alpha[poff[b] + j];
+          const double p = ea + eb;
+          const double Kab = std::exp(-ea * eb / p * AB2) * coef[poff[a] + i] * coef[poff[b] + j];
+          // primitive-pair bound sqrt((pp|pp)) of the s-type part
+          const double qp = std::fabs(Kab) * std::sqrt(2.0 * std::pow(pi, 2.5) / (p * p * std::sqrt(2.0 * p)));
+          if (qp < c->prim_threshold) continue;
+          PrimPair pp;
+          pp.p = p;
+          pp.K = Kab / p;
+          for (int k = 0; k < 3; ++k) {
+            pp.P[k] = (ea * A[k] + eb * B[k]) / p;
+            pp.PA[k] = pp.P[k] - A[k];
+            pp.PB[k] = pp.P[k] - B[k];
+          }
+          pp.o2p = 0.5 / p;
+          h.prim.push_back(pp);
+        }
+      hp.push_back(std::move(h));
+    }
+  // group by class (stable)
+  std::stable_sort(hp.begin(), hp.end(), [](const HostPair& x, const HostPair& y) { return x.cls < y.cls; });
+  c->class_start.assign(c->npc + 1, 0);
+  for (auto& h : hp) c->class_start[h.cls + 1]++;
+  for (int k = 0; k < c->npc; ++k) c->class_start[k + 1] += c->class_start[k];
+  auto flatten = [&]() {
+    c->pairs.clear();
+    c->prims.clear();
+    for (auto& h : hp) {
+      PairInfo pi_ = h.info;
+      pi_.p0 = (int)c->prims.size();
+      c->prims.insert(c->prims.end(), h.prim.begin(), h.prim.end());
+      pi_.p1 = (int)c->prims.size();
+      c->pairs.push_back(pi_);
+    }
+  };
+  flatten();
+  const int npair = (int)c->pairs.size();
+  // device uploads
+  c->d_shells.upload(c->shells, c->stream);
+  c->d_cart2shell.upload(c->cart2shell, c->stream);
+  c->d_sph2shell.upload(c->sph2shell, c->stream);
+  c->d_tco.upload(c->tco, c->stream);
+  c->d_tco_off.upload(c->tco_off, c->stream);
+  c->d_boys.upload(boys_table(), c->stream);
+  c->d_pairs.upload(c->pairs, c->stream);
+  c->d_prims.upload(c->prims, c->stream);
+  c->d_pairQ.ensure(npair);
+  // Schwarz bounds: diagonal quartets (P|P) per class on the device
+  for (int k = 0; k < kNumClassPairs; ++k) {
+    const ClassKernels& ck = kClassKernels[k];
+    if (ck.i != ck.j) continue;
+    const int X = ck.i;
+    const int n = c->class_start[X + 1] - c->class_start[X];
+    if (n == 0) continue;
+    int la = 0;
+    while ((la + 1) * (la + 2) / 2 <= X) ++la;
+    const int lb = X - la * (la + 1) / 2;
+    const int nab = ncart_l(la) * ncart_l(lb);
+    std::vector<uint2> tasks(n);
+    std::vector<int> ids(n);
+    for (int q = 0; q < n; ++q) {
+      tasks[q] = make_uint2(c->class_start[X] + q, c->class_start[X] + q);
+      ids[q] = c->class_start[X] + q;
+    }
+    DevBuf<uint2> dt;
+    DevBuf<int> di;
+    DevBuf<double> dblk;
+    dt.upload(tasks, c->stream);
+    di.upload(ids, c->stream);
+    dblk.ensure((size_t)n * nab * nab);
+    hipLaunchKernelGGL(ck.block, dim3(grid_for(n, WG)), dim3(WG), 0, c->stream, dt.p, (long)n, c->d_pairs.p,
+                       c->d_prims.p, c->d_boys.p, dblk.p, nab * nab);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(schwarz_kernel, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, dblk.p, di.p, n,
+                       c->d_pairs.p, c->d_shells.p, c->d_tco.p, c->d_tco_off.p, c->d_pairQ.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dt.release(); di.release(); dblk.release();
+  }
+  c->pairQ.resize(npair);
+  HIPCHK(hipMemcpy(c->pairQ.data(), c->d_pairQ.p, npair * sizeof(double), hipMemcpyDeviceToHost));
+  // sort within class by Schwarz bound, descending (ties: canonical order)
+  std::vector<int> order(npair);
+  std::iota(order.begin(), order.end(), 0);
+  for (int X = 0; X < c->npc; ++X)
+    std::stable_sort(order.begin() + c->class_start[X], order.begin() + c->class_start[X + 1],
+                     [&](int x, int y) { return c->pairQ[x] > c->pairQ[y]; });
+  std::vector<HostPair> hp2;
+  std::vector<double> q2;
+  hp2.reserve(npair);
+  for (int k : order) { hp2.push_back(std::move(hp[k])); q2.push_back(c->pairQ[k]); }
+  hp.swap(hp2);
+  c->pairQ.swap(q2);
+  flatten();
+  c->d_pairs.upload(c->pairs, c->stream);
+  c->d_prims.upload(c->prims, c->stream);
+  c->d_pairQ.upload(c->pairQ, c->stream);
+  c->Q.assign((size_t)nshell * nshell, 0.0);
+  for (int k = 0; k < npair; ++k) {
+    const PairInfo& pi_ = c->pairs[k];
+    c->Q[(size_t)pi_.s1 * nshell + pi_.s2] = c->pairQ[k];
+    c->Q[(size_t)pi_.s2 * nshell + pi_.s1] = c->pairQ[k];
+  }
+  // class-pair metadata and task capacity
+  c->cps.clear();
+  c->item_start.clear();
+  long long off = 0;
+  int items = 0;
+  for (int k = 0; k < kNumClassPairs; ++k) {
+    const ClassKernels& ck = kClassKernels[k];
+    CPMeta m{};
+    m.bra_start = c->class_start[ck.bi];
+    m.nbra = c->class_start[ck.bi + 1] - c->class_start[ck.bi];
+    m.ket_start = c->class_start[ck.ki];
+    m.nket = c->class_start[ck.ki + 1] - c->class_start[ck.ki];
+    m.same = ck.i == ck.j;
+    m.task_off = off;
+    const long long cap = m.same ? (long long)m.nbra * (m.nbra + 1) / 2 : (long long)m.nbra * m.nket;
+    off += cap;
+    c->cps.push_back(m);
+    c->item_start.push_back(items);
+    items += m.nbra;
+  }
+  c->nitem = items;
+  c->task_capacity = off;
+  c->d_cps.upload(c->cps, c->stream);
+  c->d_item_start.upload(c->item_start, c->stream);
+  c->d_tasks.ensure((size_t)std::max<long long>(1, off));
+  c->d_counts.ensure(kNumClassPairs);
+  c->d_nglob.ensure(1);
+  c->d_Nmax.ensure((size_t)nshell * nshell);
+  c->last_counts.assign(kNumClassPairs, 0);
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+struct Slot {
+  const double* X;
+  int do_j = 0, do_k = 0;
+};
+
+// The contraction on device-resident X/AX (column-major, leading dimension ld).
+void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const* AX, const int* types,
+                     const double* scales, int n, int ld, hipStream_t s) {
+  if (nmat < 1) throw std::invalid_argument("nmat must be >= 1");
+  if (n != c->nsph) throw std::invalid_argument("matrix dimension does not match the basis");
+  if (ld < n) throw std::invalid_argument("leading dimension smaller than the matrix dimension");
+  for (int i = 0; i < nmat; ++i) {
+    if (types[i] != CQ_COULOMB && types[i] != CQ_EXCHANGE)
+      throw std::invalid_argument("only COULOMB and EXCHANGE contractions are supported by the direct path");
+    if (!X[i] || !AX[i]) throw std::invalid_argument("null matrix pointer");
+  }
+  // unique densities (screening uses all of them, scale == 0 included)
+  std::vector<const double*> uniq;
+  for (int i = 0; i < nmat; ++i)
+    if (std::find(uniq.begin(), uniq.end(), X[i]) == uniq.end()) uniq.push_back(X[i]);
+  if ((int)uniq.size() > MAX_MATS) throw std::invalid_argument("too many distinct input matrices");
+  std::vector<Slot> slots;
+  std::vector<int> mat_slot(nmat, -1);
+  for (int i = 0; i < nmat; ++i) {
+    if (scales[i] == 0.0) continue;
+    int k = 0;
+    while (k < (int)slots.size() && slots[k].X != X[i]) ++k;
+    if (k == (int)slots.size()) slots.push_back(Slot{X[i]});
+    if (types[i] == CQ_COULOMB) slots[k].do_j = 1; else slots[k].do_k = 1;
+    mat_slot[i] = k;
+  }
+  const int ns = c->nshell;
+  const long nc2 = (long)c->ncart * c->ncart;
+  const int nslot = (int)slots.size();
+  // workspace: per slot Dc, GJ, GJ2, GK, GK2
+  c->d_work.ensure((size_t)std::max(1, nslot) * 5 * nc2);
+  c->d_flags.ensure(std::max(1, nslot));
+  // 1. block norms over every input matrix
+  MatList ml{};
+  ml.n = (int)uniq.size();
+  for (int i = 0; i < ml.n; ++i) ml.X[i] = uniq[i];
+  HIPCHK(hipMemsetAsync(c->d_nglob.p, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(norm_kernel, dim3(grid_for((long)ns * ns, WG)), dim3(WG), 0, s, ml, ld, c->d_shells.p, ns,
+                     c->d_Nmax.p, c->d_nglob.p);
+  HIPCHK(hipGetLastError());
+  if (nslot == 0) return;
+  // 2. Cartesian densities, symmetry flags, zeroed accumulators
+  //    per slot: [Dc | GJ | GJ2 | GK | GK2], each ncart^2
+  HIPCHK(hipMemsetAsync(c->d_flags.p, 0, nslot * sizeof(int), s));
+  for (int k = 0; k < nslot; ++k) {
+    double* base = c->d_work.p + (size_t)k * 5 * nc2;
+    hipLaunchKernelGGL(cart_density_kernel, dim3(grid_for(nc2, WG)), dim3(WG), 0, s, slots[k].X, ld,
+                       c->d_shells.p, c->d_cart2shell.p, c->ncart, c->d_tco.p, c->d_tco_off.p, base);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(nonsym_kernel, dim3(grid_for((long)n * n, WG)), dim3(WG), 0, s, slots[k].X, n, ld,
+                       c->d_flags.p + k);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(base + nc2, 0, 4 * nc2 * sizeof(double), s));
+  }
+  // 3. screening into compacted task lists
+  HIPCHK(hipMemsetAsync(c->d_counts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
+  if (c->nitem > 0) {
+    hipLaunchKernelGGL(screen_kernel, dim3(grid_for(c->nitem, WG / 64)), dim3(WG), 0, s, c->d_cps.p,
+                       kNumClassPairs, c->d_item_start.p, c->nitem, c->d_pairs.p, c->d_pairQ.p, c->d_Nmax.p,
+                       c->d_nglob.p, ns, c->tau, c->d_tasks.p, c->d_counts.p);
+    HIPCHK(hipGetLastError());
+  }
+  // 4. ERI + J/K digestion, class pair by class pair, two density slots per pass
+  for (int s0 = 0; s0 < nslot; s0 += MAX_SLOTS) {
+    const int ng = std::min(MAX_SLOTS, nslot - s0);
+    JKArgs jk{};
+    jk.ncart = c->ncart;
+    for (int g = 0; g < ng; ++g) {
+      double* base = c->d_work.p + (size_t)(s0 + g) * 5 * nc2;
+      jk.D[g] = base;
+      jk.GJ[g] = base + nc2;
+      jk.GJ2[g] = base + 2 * nc2;
+      jk.GK[g] = base + 3 * nc2;
+      jk.GK2[g] = base + 4 * nc2;
+      jk.do_j[g] = slots[s0 + g].do_j;
+      jk.do_k[g] = slots[s0 + g].do_k;
+    }
+    jk.nonsym = c->d_flags.p + s0;
+    for (int k = 0; k < kNumClassPairs; ++k) {
+      const ClassKernels& ck = kClassKernels[k];
+      if (c->cps[k].nbra == 0 || c->cps[k].nket == 0) continue;
+      const uint2* tasks = c->d_tasks.p + c->cps[k].task_off;
+      const long long cap = c->cps[k].same ? (long long)c->cps[k].nbra * (c->cps[k].nbra + 1) / 2
+                                           : (long long)c->cps[k].nbra * c->cps[k].nket;
+      const int grid = (int)std::min<long long>(persistent_grid(), (cap + WG - 1) / WG);
+      JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
+      hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, s, tasks, c->d_counts.p + k, c->d_pairs.p,
+                         c->d_prims.p, c->d_boys.p, jk);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  // 5. back to the shells' basis and accumulate: AX += 1/2 scale (G + G^T-partner)
+  for (int i = 0; i < nmat; ++i) {
+    const int k = mat_slot[i];
+    if (k < 0) continue;
+    double* base = c->d_work.p + (size_t)k * 5 * nc2;
+    const double f = 0.5 * scales[i];
+    // COULOMB: G = GJ + (nonsym ? GJ2 : GJ)^T ; EXCHANGE: G = GK + (nonsym ? GK2 : GK)^T
+    const double* G1 = base + (types[i] == CQ_COULOMB ? 1 : 3) * nc2;
+    const double* G2 = base + (types[i] == CQ_COULOMB ? 2 : 4) * nc2;
+    hipLaunchKernelGGL(finalize_kernel, dim3(grid_for((long)n * n, WG)), dim3(WG), 0, s, AX[i], n, ld, f, G1, G1,
+                       c->d_flags.p + k, G2, c->d_shells.p, c->d_sph2shell.p, c->ncart, c->d_tco.p,
+                       c->d_tco_off.p);
+    HIPCHK(hipGetLastError());
+  }
+}
+
+template <typename F>
+int guarded(cq_ctx* c, F&& f) {
+  if (!c) return CQ_ERR_INVALID;
+  try {
+    f();
+    c->err.clear();
+    return CQ_OK;
+  } catch (const std::invalid_argument& e) {
+    c->err = e.what();
+    return CQ_ERR_INVALID;
+  } catch (const std::bad_alloc& e) {
+    c->err = "out of memory";
+    return CQ_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return CQ_ERR_DEVICE;
+  }
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+// =======================================================================================
+// C ABI
+// =======================================================================================
+extern "C" {
+
+int cq_ctx_create(cq_ctx_t** out, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                  const double* coef, const double* origin) {
+  if (!out) return CQ_ERR_INVALID;
+  *out = nullptr;
+  if (nshell < 1 || !l || !pure || !nprim || !alpha || !coef || !origin) {
+    g_create_error = "invalid shell description";
+    return CQ_ERR_INVALID;
+  }
+  cq_ctx* c = new (std::nothrow) cq_ctx();
+  if (!c) return CQ_ERR_NOMEM;
+  try {
+    build_context(c, nshell, l, pure, nprim, alpha, coef, origin);
+  } catch (const std::invalid_argument& e) {
+    g_create_error = e.what();
+    delete c;
+    return CQ_ERR_INVALID;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    delete c;
+    return CQ_ERR_DEVICE;
+  }
+  *out = c;
+  return CQ_OK;
+}
+
+int cq_ctx_destroy(cq_ctx_t* c) {
+  if (!c) return CQ_ERR_INVALID;
+  (void)hipStreamSynchronize(c->stream);
+  delete c;
+  return CQ_OK;
+}
+
+const char* cq_last_error(const cq_ctx_t* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int cq_set_schwarz_threshold(cq_ctx_t* c, double tau) {
+  return guarded(c, [&] {
+    if (!(tau >= 0.0)) throw std::invalid_argument("threshold must be non-negative");
+    c->tau = tau;
+  });
+}
+
+int cq_get_schwarz_threshold(const cq_ctx_t* c, double* tau) {
+  if (!c || !tau) return CQ_ERR_INVALID;
+  *tau = c->tau;
+  return CQ_OK;
+}
+
+int cq_set_precision_mode(cq_ctx_t* c, int mode, double threshold) {
+  return guarded(c, [&] {
+    if (mode != CQ_PREC_FP64) throw std::invalid_argument("only CQ_PREC_FP64 is available");
+    (void)threshold;
+    c->precision_mode = mode;
+  });
+}
+
+int cq_get_dims(const cq_ctx_t* c, int* nshell, int* nbf, int* ncart, int* npair, int* nprim_pair) {
+  if (!c) return CQ_ERR_INVALID;
+  if (nshell) *nshell = c->nshell;
+  if (nbf) *nbf = c->nsph;
+  if (ncart) *ncart = c->ncart;
+  if (npair) *npair = (int)c->pairs.size();
+  if (nprim_pair) *nprim_pair = (int)c->prims.size();
+  return CQ_OK;
+}
+
+int cq_get_schwarz(const cq_ctx_t* c, double* Q) {
+  if (!c || !Q) return CQ_ERR_INVALID;
+  std::memcpy(Q, c->Q.data(), c->Q.size() * sizeof(double));
+  return CQ_OK;
+}
+
+int cq_contract_direct(cq_ctx_t* c, int nmat, const double* const* X, double* const* AX, const int* types,
+                       const double* scales, int n, int ld) {
+  return guarded(c, [&] {
+    if (nmat < 1 || nmat > MAX_MATS) throw std::invalid_argument("nmat out of range");
+    if (ld < n) throw std::invalid_argument("leading dimension smaller than the matrix dimension");
+    const size_t bytes = (size_t)ld * n * sizeof(double);
+    c->d_mats.ensure((size_t)2 * nmat * ld * n);
+    std::vector<const double*> dX(nmat);
+    std::vector<double*> dAX(nmat);
+    for (int i = 0; i < nmat; ++i) {
+      if (!X[i] || !AX[i]) throw std::invalid_argument("null matrix pointer");
+      double* px = c->d_mats.p + (size_t)(2 * i) * ld * n;
+      double* pa = c->d_mats.p + (size_t)(2 * i + 1) * ld * n;
+      // identical host buffers map to one device buffer (same density, screened once)
+      int dup = -1;
+      for (int k = 0; k < i; ++k) if (X[k] == X[i]) dup = k;
+      if (dup >= 0) px = const_cast<double*>(dX[dup]);
+      else HIPCHK(hipMemcpyAsync(px, X[i], bytes, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(pa, AX[i], bytes, hipMemcpyHostToDevice, c->stream));
+      dX[i] = px;
+      dAX[i] = pa;
+    }
+    contract_device(c, nmat, dX.data(), dAX.data(), types, scales, n, ld, c->stream);
+    for (int i = 0; i < nmat; ++i)
+      HIPCHK(hipMemcpyAsync(AX[i], dAX[i], bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int cq_contract_direct_device(cq_ctx_t* c, int nmat, const double* const* dX, double* const* dAX,
+                              const int* types, const double* scales, int n, int ld, void* stream) {
+  return guarded(c, [&] {
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    contract_device(c, nmat, dX, dAX, types, scales, n, ld, s);
+  });
+}
+
+int cq_get_quartet_counts(cq_ctx_t* c, long long* counts, int max_counts, int* n_out) {
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<unsigned long long> h(kNumClassPairs);
+    HIPCHK(hipMemcpy(h.data(), c->d_counts.p, kNumClassPairs * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kNumClassPairs && k < max_counts; ++k) counts[k] = (long long)h[k];
+    if (n_out) *n_out = kNumClassPairs;
+  });
+}
+
+int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld_) {
+  if (k < 0 || k >= kNumClassPairs) return CQ_ERR_INVALID;
+  const ClassKernels& ck = kClassKernels[k];
+  auto split = [](int X, int* a, int* b) {
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= X) ++l;
+    *a = l;
+    *b = X - l * (l + 1) / 2;
+  };
+  split(ck.bi, la, lb);
+  split(ck.ki, lc, ld_);
+  return CQ_OK;
+}
+
+int cq_num_class_pairs(void) { return kNumClassPairs; }
+
+// Contracted quartets (s1 s2|s3 s4) in the shells' own basis, row-major per quartet,
+// concatenated; for per-quartet parity tests.
+int cq_debug_quartets(cq_ctx_t* c, int nq, const int* shells4, double* out) {
+  return guarded(c, [&] {
+    // map canonical shell pair -> pair index
+    std::vector<int> pidx((size_t)c->nshell * (c->nshell + 1) / 2, -1);
+    for (int k = 0; k < (int)c->pairs.size(); ++k) {
+      const PairInfo& p = c->pairs[k];
+      pidx[(size_t)p.s1 * (p.s1 + 1) / 2 + p.s2] = k;
+    }
+    size_t out_off = 0;
+    for (int q = 0; q < nq; ++q) {
+      int s[4] = {shells4[4 * q], shells4[4 * q + 1], shells4[4 * q + 2], shells4[4 * q + 3]};
+      for (int k = 0; k < 4; ++k)
+        if (s[k] < 0 || s[k] >= c->nshell) throw std::invalid_argument("shell index out of range");
+      auto pid = [&](int x, int y) { return pidx[(size_t)std::max(x, y) * (std::max(x, y) + 1) / 2 + std::min(x, y)]; };
+      const int P = pid(s[0], s[1]), Qi = pid(s[2], s[3]);
+      const PairInfo& pp = c->pairs[P];
+      const PairInfo& pq = c->pairs[Qi];
+      const int clsP = pair_class(c->shells[pp.a].l, c->shells[pp.b].l);
+      const int clsQ = pair_class(c->shells[pq.a].l, c->shells[pq.b].l);
+      int kk = -1;
+      bool swap_bk = false;
+      for (int k = 0; k < kNumClassPairs; ++k) {
+        if (kClassKernels[k].bi == clsP && kClassKernels[k].ki == clsQ) { kk = k; swap_bk = false; break; }
+        if (kClassKernels[k].bi == clsQ && kClassKernels[k].ki == clsP) { kk = k; swap_bk = true; break; }
+      }
+      if (kk < 0) throw std::runtime_error("no kernel for quartet class");
+      const int bra = swap_bk ? Qi : P, ket = swap_bk ? P : Qi;
+      const PairInfo& B = c->pairs[bra];
+      const PairInfo& K = c->pairs[ket];
+      const int na = ncart_l(c->shells[B.a].l), nb = ncart_l(c->shells[B.b].l);
+      const int nc = ncart_l(c->shells[K.a].l), nd = ncart_l(c->shells[K.b].l);
+      const int blk = na * nb * nc * nd;
+      std::vector<uint2> t1{make_uint2(bra, ket)};
+      DevBuf<uint2> dt;
+      DevBuf<double> dout;
+      dt.upload(t1, c->stream);
+      dout.ensure(blk);
+      hipLaunchKernelGGL(kClassKernels[kk].block, dim3(1), dim3(WG), 0, c->stream, dt.p, 1L, c->d_pairs.p,
+                         c->d_prims.p, c->d_boys.p, dout.p, blk);
+      HIPCHK(hipGetLastError());
+      std::vector<double> h(blk);
+      HIPCHK(hipMemcpyAsync(h.data(), dout.p, blk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      dt.release();
+      dout.release();
+      // kernel order (B.a B.b | K.a K.b) -> requested (s0 s1 | s2 s3), Cartesian
+      int kshell[4] = {B.a, B.b, K.a, K.b};
+      int kdim[4] = {na, nb, nc, nd};
+      int perm[4];   // requested position r -> kernel position
+      bool used[4] = {false, false, false, false};
+      // bra/ket of the request map to the pair that holds them
+      const int rb0 = swap_bk ? 2 : 0;   // kernel positions holding the request's bra pair
+      const int rk0 = swap_bk ? 0 : 2;
+      auto assign = [&](int r, int base) {
+        for (int k2 = base; k2 < base + 2; ++k2)
+          if (!used[k2] && kshell[k2] == s[r]) { perm[r] = k2; used[k2] = true; return; }
+      };
+      assign(0, rb0); assign(1, rb0); assign(2, rk0); assign(3, rk0);
+      int rdim[4];
+      for (int r = 0; r < 4; ++r) rdim[r] = kdim[perm[r]];
+      std::vector<double> cartq((size_t)blk);
+      int idx[4];
+      for (idx[0] = 0; idx[0] < rdim[0]; ++idx[0])
+        for (idx[1] = 0; idx[1] < rdim[1]; ++idx[1])
+          for (idx[2] = 0; idx[2] < rdim[2]; ++idx[2])
+            for (idx[3] = 0; idx[3] < rdim[3]; ++idx[3]) {
+              int kidx[4];
+              for (int r = 0; r < 4; ++r) kidx[perm[r]] = idx[r];
+              const size_t ksrc = (((size_t)kidx[0] * nb + kidx[1]) * nc + kidx[2]) * nd + kidx[3];
+              const size_t rdst = (((size_t)idx[0] * rdim[1] + idx[1]) * rdim[2] + idx[2]) * rdim[3] + idx[3];
+              cartq[rdst] = h[ksrc];
+            }
+      // Cartesian -> shells' basis, index by index
+      std::vector<double> cur = cartq;
+      int dims[4] = {rdim[0], rdim[1], rdim[2], rdim[3]};
+      for (int r = 0; r < 4; ++r) {
+        const ShellMeta& sm = c->shells[s[r]];
+        if (!sm.pure) continue;
+        const int ncs = ncart_l(sm.l), nss = 2 * sm.l + 1;
+        const double* M = c->tco.data() + c->tco_off[sm.l];
+        size_t pre = 1, post = 1;
+        for (int k2 = 0; k2 < r; ++k2) pre *= dims[k2];
+        for (int k2 = r + 1; k2 < 4; ++k2) post *= dims[k2];
+        std::vector<double> nx(pre * nss * post, 0.0);
+        for (size_t p0 = 0; p0 < pre; ++p0)
+          for (int i = 0; i < nss; ++i)
+            for (int a = 0; a < ncs; ++a) {
+              const double w = M[i * ncs + a];
+              if (w == 0.0) continue;
+              for (size_t p1 = 0; p1 < post; ++p1)
+                nx[(p0 * nss + i) * post + p1] += w * cur[(p0 * ncs + a) * post + p1];
+            }
+        cur.swap(nx);
+        dims[r] = nss;
+      }
+      std::memcpy(out + out_off, cur.data(), cur.size() * sizeof(double));
+      out_off += cur.size();
+    }
+  });
+}
+
+}  // extern "C"

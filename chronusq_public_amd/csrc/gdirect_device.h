@@ -1,0 +1,135 @@
+// Device-side types and helpers shared by the generated per-class ERI cores and the
+// G[D] kernels.  gfx950 only (64-wide wavefronts, FP64 VALU, FP64 global atomics).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Boys-function table: F_m(T_i) on T_i = i * BOYS_STEP, i = 0..BOYS_NGRID-1, for
+// m = 0..BOYS_MTAB-1, plus exp(-T_i) in the last column.  Each kernel stages the
+// slice m = L..L+7 and the exp column (9 doubles per grid point) into LDS.
+#define BOYS_STEP 0.1
+#define BOYS_INV_STEP 10.0
+#define BOYS_TMAX 36.0
+#define BOYS_NGRID 361
+#define BOYS_MTAB 24
+#define BOYS_LDS_W 9
+
+#define TWO_PI_52 34.986836655249725693   // 2 * pi^(5/2)
+
+// One primitive pair of a shell pair (a, b):
+//   p = alpha_a + alpha_b, K = exp(-alpha_a alpha_b / p |AB|^2) c_a c_b / p,
+//   P = (alpha_a A + alpha_b B) / p, PA = P - A, PB = P - B, o2p = 1 / (2p).
+struct alignas(16) PrimPair {
+  double p;
+  double K;
+  double P[3];
+  double PA[3];
+  double PB[3];
+  double o2p;
+};
+
+// Per shell pair, in kernel orientation (l_a >= l_b).
+struct PairInfo {
+  int a, b;        // shells in kernel orientation
+  int s1, s2;      // canonical shells (s1 >= s2), as the reference loop visits them
+  int p0, p1;      // primitive-pair range in the PrimPair array
+  int a0, b0;      // first Cartesian function of a and b
+};
+
+struct QuartetArgs {
+  const PrimPair* __restrict__ prim;
+  int bra_p0, bra_p1, ket_p0, ket_p1;
+  int a0, b0, c0, d0;
+  double deg;
+  bool bra_uniform;
+  bool bra_swap, ket_swap;   // pair held reversed relative to its canonical (s1 >= s2) order
+};
+
+#define MAX_SLOTS 2
+struct JKArgs {
+  int ncart;
+  const double* D[MAX_SLOTS];    // density in the Cartesian basis, row-major ncart x ncart
+  double* GJ[MAX_SLOTS];         // forward Coulomb updates
+  double* GJ2[MAX_SLOTS];        // forward Coulomb updates from the transposed density
+  double* GK[MAX_SLOTS];         // forward exchange updates
+  double* GK2[MAX_SLOTS];        // forward exchange updates from the transposed density
+  int do_j[MAX_SLOTS];
+  int do_k[MAX_SLOTS];
+  const int* nonsym;             // device flags: density slot is not symmetric
+};
+
+// 1/sqrt(x) for x > 0: hardware estimate (about 2^-23 relative) + two Newton steps.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double d = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, d, y);
+  d = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, d, y);
+  return y;
+}
+
+// F_0..F_L at T.  T < 36: 8-term Taylor expansion of F_L about the nearest grid point
+// (|dT| <= 0.05, truncation below 1e-15 relative), exp(-T) = exp(-T0) exp(dT) by an
+// 8-term series, downward recursion F_m = (2T F_{m+1} + e^-T) / (2m+1).
+// T >= 36: F_0 = sqrt(pi/T)/2, upward recursion F_{m+1} = (2m+1)/(2T) F_m (the dropped
+// e^-T term is below 2.4e-16).
+template <int L>
+__device__ __forceinline__ void boys_eval(double T, const double* __restrict__ lds, double* F) {
+  if (T < BOYS_TMAX) {
+    const int i = (int)(T * BOYS_INV_STEP + 0.5);
+    const double dT = (double)i * BOYS_STEP - T;
+    const double* f = lds + i * BOYS_LDS_W;
+    double s = f[7];
+    s = f[6] + s * dT * (1.0 / 7.0);
+    s = f[5] + s * dT * (1.0 / 6.0);
+    s = f[4] + s * dT * (1.0 / 5.0);
+    s = f[3] + s * dT * (1.0 / 4.0);
+    s = f[2] + s * dT * (1.0 / 3.0);
+    s = f[1] + s * dT * (1.0 / 2.0);
+    s = f[0] + s * dT;
+    F[L] = s;
+    if constexpr (L > 0) {
+      double e = 1.0 + dT * (1.0 / 8.0);
+      e = 1.0 + dT * (1.0 / 7.0) * e;
+      e = 1.0 + dT * (1.0 / 6.0) * e;
+      e = 1.0 + dT * (1.0 / 5.0) * e;
+      e = 1.0 + dT * (1.0 / 4.0) * e;
+      e = 1.0 + dT * (1.0 / 3.0) * e;
+      e = 1.0 + dT * (1.0 / 2.0) * e;
+      e = 1.0 + dT * e;
+      e *= f[8];
+      const double T2 = 2.0 * T;
+#pragma unroll
+      for (int m = L - 1; m >= 0; --m) F[m] = (T2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+    }
+  } else {
+    const double rs = rsqrt_f64(T);
+    F[0] = 0.88622692545275801365 * rs;   // sqrt(pi)/2
+    if constexpr (L > 0) {
+      const double hr = 0.5 * rs * rs;
+#pragma unroll
+      for (int m = 0; m < L; ++m) F[m + 1] = F[m] * (double)(2 * m + 1) * hr;
+    }
+  }
+}
+
+// Stage the Boys slice for total angular momentum L into LDS (whole workgroup).
+__device__ __forceinline__ void stage_boys(const double* __restrict__ tab, int L, double* lds) {
+  for (int k = threadIdx.x; k < BOYS_NGRID * BOYS_LDS_W; k += blockDim.x) {
+    const int i = k / BOYS_LDS_W, c = k - i * BOYS_LDS_W;
+    lds[k] = (c < 8) ? tab[i * (BOYS_MTAB + 1) + L + c] : tab[i * (BOYS_MTAB + 1) + BOYS_MTAB];
+  }
+  __syncthreads();
+}
+
+// Add a J_ab contribution; when the whole wavefront shares one bra pair the 64 values are
+// summed across lanes first and one lane issues the atomic.
+__device__ __forceinline__ void jab_add(double* p, double v, bool uniform) {
+  if (uniform) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(p, v);
+  } else {
+    atomicAdd(p, v);
+  }
+}

@@ -1,0 +1,547 @@
+#!/usr/bin/env python3
+"""Generate the per-class McMurchie-Davidson ERI kernels for gfx950.
+
+One HIP device function per shell-quartet class (la lb | lc ld), fully unrolled: the
+Hermite expansion coefficients E (bra and ket), the Hermite Coulomb integrals R_tuv, the
+ket Hermite->Cartesian transform and the bra transform are straight-line code over
+scalars, so every intermediate lives in a register.  The ket Cartesian components are
+processed in chunks sized to keep the live set under a register budget; each chunk is
+digested into J and K (forward updates) before the next one starts.
+
+Two kernels are instantiated per class from the generated core:
+  * eri_block_<cls>: writes the contracted Cartesian block (ab|cd) of each task (used for
+    Schwarz bounds and for per-quartet parity tests);
+  * eri_jk_<cls>: digests the block into the forward J/K accumulators.
+
+The algorithm restated here replaces libint2::Engine::compute as called at
+include/aointegrals/aointegrals_contract.h:138-143; the digestion reproduces the scatter at
+include/aointegrals/aointegrals_contract.h:145-171 (see the J/K notes in gdirect.hip).
+
+Usage: gen_eri.py OUTPUT_DIR
+"""
+from __future__ import annotations
+
+import sys
+from typing import Dict, List, Tuple
+
+CLS_NAME = "spdfg"
+
+
+def cart(l: int) -> List[Tuple[int, int, int]]:
+    out = []
+    for i in range(l + 1):
+        lx = l - i
+        for j in range(i + 1):
+            ly = i - j
+            out.append((lx, ly, l - lx - ly))
+    return out
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def hermite_list(L):
+    """(t,u,v) with t+u+v <= L."""
+    out = []
+    for n in range(L + 1):
+        for t in range(n, -1, -1):
+            for u in range(n - t, -1, -1):
+                out.append((t, u, n - t - u))
+    return out
+
+
+class Emitter:
+    def __init__(self):
+        self.lines: List[str] = []
+        self.ind = 1
+
+    def __call__(self, s: str):
+        self.lines.append("  " * self.ind + s)
+
+
+# --------------------------------------------------------------------------- E coefficients
+def gen_E(em: Emitter, pre: str, d: str, LA: int, LB: int, PA: str, PB: str, o2p: str, sign: bool) -> Dict:
+    """Emit E^{ij}_t for one direction; returns dict (i,j,t) -> expression string
+    (a variable name or a literal).  sign=True folds (-1)^t into the result (ket side)."""
+    E: Dict[Tuple[int, int, int], str] = {(0, 0, 0): "1.0"}
+
+    def term(coef: str, e: str):
+        if e == "0" or coef == "0":
+            return None
+        if e == "1.0":
+            return coef
+        if coef == "1.0":
+            return e
+        return f"{coef}*{e}"
+
+    for i in range(LA + 1):
+        for j in range(LB + 1):
+            if i == 0 and j == 0:
+                continue
+            if i > 0:
+                si, sj, X = i - 1, j, PA
+            else:
+                si, sj, X = i, j - 1, PB
+            for t in range(i + j + 1):
+                parts = []
+                if t > 0 and (si, sj, t - 1) in E:
+                    parts.append(term(o2p, E[(si, sj, t - 1)]))
+                if (si, sj, t) in E:
+                    parts.append(term(X, E[(si, sj, t)]))
+                if (si, sj, t + 1) in E:
+                    parts.append(term(f"{float(t + 1)}", E[(si, sj, t + 1)]))
+                parts = [p for p in parts if p is not None]
+                if not parts:
+                    continue
+                expr = " + ".join(parts)
+                name = f"{pre}{d}_{i}_{j}_{t}"
+                # plain aliases need no variable
+                if len(parts) == 1 and ("*" not in parts[0]) and ("+" not in parts[0]):
+                    E[(i, j, t)] = parts[0]
+                else:
+                    em(f"const double {name} = {expr};")
+                    E[(i, j, t)] = name
+    if sign:
+        out = {}
+        for (i, j, t), e in E.items():
+            out[(i, j, t)] = (e, -1 if t % 2 else 1)
+        return out
+    return {k: (v, 1) for k, v in E.items()}
+
+
+def mul(*fs):
+    """Product of (expr, sign) factors -> (expr or None for 1, sign)."""
+    s = 1
+    parts = []
+    for e, sg in fs:
+        s *= sg
+        if e == "1.0":
+            continue
+        parts.append(e)
+    return ("*".join(parts) if parts else None, s)
+
+
+# --------------------------------------------------------------------------- R integrals
+class RGen:
+    def __init__(self, em: Emitter, L: int):
+        self.em = em
+        self.L = L
+        self.memo: Dict[Tuple[int, int, int, int], str] = {}
+
+    def get(self, n, t, u, v) -> str:
+        key = (n, t, u, v)
+        if key in self.memo:
+            return self.memo[key]
+        if t == 0 and u == 0 and v == 0:
+            name = f"Rb{n}"
+            self.memo[key] = name
+            return name
+        if t > 0:
+            a = self.get(n + 1, t - 1, u, v)
+            b = self.get(n + 1, t - 2, u, v) if t > 1 else None
+            X, k = "PQx", t - 1
+        elif u > 0:
+            a = self.get(n + 1, t, u - 1, v)
+            b = self.get(n + 1, t, u - 2, v) if u > 1 else None
+            X, k = "PQy", u - 1
+        else:
+            a = self.get(n + 1, t, u, v - 1)
+            b = self.get(n + 1, t, u, v - 2) if v > 1 else None
+            X, k = "PQz", v - 1
+        name = f"R{n}_{t}_{u}_{v}"
+        if b is not None:
+            kk = f"{float(k)}"
+            expr = f"{X}*{a} + {b}" if k == 1 else f"{X}*{a} + {kk}*{b}"
+        else:
+            expr = f"{X}*{a}"
+        self.em(f"const double {name} = {expr};")
+        self.memo[key] = name
+        return name
+
+
+# --------------------------------------------------------------------------- class core
+def chunk_plan(LA, LB, LC, LD, budget=96):
+    NAB = ncart(LA) * ncart(LB)
+    HAB = len(hermite_list(LA + LB))
+    ND = ncart(LD)
+    NC = ncart(LC)
+    HL = len(hermite_list(LA + LB + LC + LD))
+    persist = NAB + ncart(LA) * ND + ncart(LB) * ND
+    cs = NC
+    while cs > 1 and (NAB + HAB) * cs * ND + HL + persist + 24 > budget:
+        cs -= 1
+    chunks = []
+    c = 0
+    while c < NC:
+        chunks.append(list(range(c, min(NC, c + cs))))
+        c += cs
+    return chunks
+
+
+def gen_class(LA, LB, LC, LD, kind: str) -> str:
+    """kind: 'jk' or 'block'."""
+    name = f"{CLS_NAME[LA]}{CLS_NAME[LB]}{CLS_NAME[LC]}{CLS_NAME[LD]}"
+    L = LA + LB + LC + LD
+    ca, cb, cc, cd = cart(LA), cart(LB), cart(LC), cart(LD)
+    NA, NB, NC, ND = len(ca), len(cb), len(cc), len(cd)
+    herm_ab = hermite_list(LA + LB)
+    chunks = chunk_plan(LA, LB, LC, LD) if kind == "jk" else [list(range(NC))]
+    em = Emitter()
+    em.ind = 0
+    em(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}) kind={kind} chunks={len(chunks)}")
+    if kind == "jk":
+        em("template <int NSLOT>")
+        em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds) {{")
+    else:
+        em(f"__device__ __forceinline__ void quartet_block_{name}(const QuartetArgs& q, double* __restrict__ out, const double* __restrict__ boys_lds) {{")
+    em.ind = 1
+    if kind == "jk":
+        # persistent digestion accumulators
+        for ia in range(NA):
+            for ib in range(NB):
+                em(f"double jab_{ia}_{ib}[NSLOT] = {{}};")
+        for ia in range(NA):
+            for idd in range(ND):
+                em(f"double kad_{ia}_{idd}[NSLOT] = {{}};")
+        for ib in range(NB):
+            for idd in range(ND):
+                em(f"double kbd_{ib}_{idd}[NSLOT] = {{}};")
+    for ci, chunk in enumerate(chunks):
+        em(f"{{ // chunk {ci}: c components {chunk}")
+        em.ind += 1
+        ks = [(ic, idd) for ic in chunk for idd in range(ND)]
+        for ia in range(NA):
+            for ib in range(NB):
+                for (ic, idd) in ks:
+                    em(f"double o_{ia}_{ib}_{ic}_{idd} = 0.0;")
+        em("for (int ip = q.bra_p0; ip < q.bra_p1; ++ip) {")
+        em.ind += 1
+        em("const PrimPair bp = q.prim[ip];")
+        Ebra = {}
+        for d, PAv, PBv in (("x", "bp.PA[0]", "bp.PB[0]"), ("y", "bp.PA[1]", "bp.PB[1]"), ("z", "bp.PA[2]", "bp.PB[2]")):
+            Ebra[d] = gen_E(em, "Eb", d, LA, LB, PAv, PBv, "bp.o2p", sign=False)
+        for (t, u, v) in herm_ab:
+            for (ic, idd) in ks:
+                em(f"double W_{t}_{u}_{v}_{ic}_{idd} = 0.0;")
+        em("for (int iq = q.ket_p0; iq < q.ket_p1; ++iq) {")
+        em.ind += 1
+        em("const PrimPair kp = q.prim[iq];")
+        em("const double pq = bp.p + kp.p;")
+        em("const double rs = rsqrt_f64(pq);")
+        em("const double rr = rs * rs;")
+        em("const double alpha = bp.p * kp.p * rr;")
+        em("const double PQx = bp.P[0] - kp.P[0];")
+        em("const double PQy = bp.P[1] - kp.P[1];")
+        em("const double PQz = bp.P[2] - kp.P[2];")
+        em("const double Targ = alpha * (PQx * PQx + PQy * PQy + PQz * PQz);")
+        em(f"double F[{L + 1}];")
+        em(f"boys_eval<{L}>(Targ, boys_lds, F);")
+        em("double pw = TWO_PI_52 * kp.K * rs;")
+        for n in range(L + 1):
+            em(f"const double Rb{n} = F[{n}] * pw;")
+            if n < L:
+                em("pw *= -2.0 * alpha;")
+        Eket = {}
+        for d, i in (("x", 0), ("y", 1), ("z", 2)):
+            Eket[d] = gen_E(em, "Ek", d, LC, LD, f"kp.PA[{i}]", f"kp.PB[{i}]", "kp.o2p", sign=True)
+        rg = RGen(em, L)
+        # C products and W updates per ket component
+        for (ic, idd) in ks:
+            (kx, ky, kz), (lx, ly, lz) = cc[ic], cd[idd]
+            terms = []
+            for tau in range(kx + lx + 1):
+                if (kx, lx, tau) not in Eket["x"]:
+                    continue
+                for nu in range(ky + ly + 1):
+                    if (ky, ly, nu) not in Eket["y"]:
+                        continue
+                    for ph in range(kz + lz + 1):
+                        if (kz, lz, ph) not in Eket["z"]:
+                            continue
+                        e, s = mul(Eket["x"][(kx, lx, tau)], Eket["y"][(ky, ly, nu)], Eket["z"][(kz, lz, ph)])
+                        cname = None
+                        if e is not None and "*" in e:
+                            cname = f"C_{ic}_{idd}_{tau}_{nu}_{ph}"
+                            em(f"const double {cname} = {'-' if s < 0 else ''}{e};")
+                            s = 1
+                        elif e is not None:
+                            cname = e
+                        terms.append((tau, nu, ph, cname, s))
+            for (t, u, v) in herm_ab:
+                parts = []
+                for (tau, nu, ph, cname, s) in terms:
+                    r = rg.get(0, t + tau, u + nu, v + ph)
+                    if cname is None:
+                        parts.append(("+" if s > 0 else "-", r))
+                    else:
+                        parts.append(("+" if s > 0 else "-", f"{cname}*{r}"))
+                expr = ""
+                for sg, p in parts:
+                    expr += f" {sg} {p}"
+                em(f"W_{t}_{u}_{v}_{ic}_{idd} +={expr};")
+        em.ind -= 1
+        em("}  // ket primitives")
+        # bra transform
+        for ia, (ax, ay, az) in enumerate(ca):
+            for ib, (bx, by, bz) in enumerate(cb):
+                bterms = []
+                for t in range(ax + bx + 1):
+                    if (ax, bx, t) not in Ebra["x"]:
+                        continue
+                    for u in range(ay + by + 1):
+                        if (ay, by, u) not in Ebra["y"]:
+                            continue
+                        for v in range(az + bz + 1):
+                            if (az, bz, v) not in Ebra["z"]:
+                                continue
+                            e, s = mul(Ebra["x"][(ax, bx, t)], Ebra["y"][(ay, by, u)], Ebra["z"][(az, bz, v)])
+                            bn = f"B_{ia}_{ib}_{t}_{u}_{v}"
+                            em(f"const double {bn} = bp.K{('*' + e) if e else ''};")
+                            bterms.append((t, u, v, bn))
+                for (ic, idd) in ks:
+                    expr = " + ".join(f"{bn}*W_{t}_{u}_{v}_{ic}_{idd}" for (t, u, v, bn) in bterms)
+                    em(f"o_{ia}_{ib}_{ic}_{idd} += {expr};")
+        em.ind -= 1
+        em("}  // bra primitives")
+        if kind == "block":
+            for ia in range(NA):
+                for ib in range(NB):
+                    for (ic, idd) in ks:
+                        idx = ((ia * NB + ib) * NC + ic) * ND + idd
+                        em(f"out[{idx}] = o_{ia}_{ib}_{ic}_{idd};")
+        else:
+            gen_digest_chunk(em, NA, NB, NC, ND, chunk)
+        em.ind -= 1
+        em("}")
+    if kind == "jk":
+        gen_digest_final(em, NA, NB, ND)
+    em.ind = 0
+    em("}")
+    return "\n".join(em.lines) + "\n"
+
+
+def _jk_addressing(em: Emitter):
+    """Index forms for the Coulomb scatter in each pair's canonical order (s1 >= s2).
+
+    The reference's updates G(1,2) += X(4,3) v, G(3,4) += X(2,1) v (and the transposed
+    G(2,1) += X(3,4) v, G(4,3) += X(1,2) v; aointegrals_contract.h:153-157) refer to the
+    canonical order inside each pair; the kernel may hold a pair reversed (bra_swap,
+    ket_swap).  Address of element (i, j) of a block = base + i*stride_i + j*stride_j with
+    i the kernel-order a (or c) component and j the b (or d) component:
+      tab: (1,2) = (s1,s2) of the bra   xba: (2,1)
+      tcd: (3,4) = (s3,s4) of the ket   xdc: (4,3)"""
+    em("const int N = jk.ncart;")
+    em("const int tab = q.bra_swap ? q.b0 * N + q.a0 : q.a0 * N + q.b0;")
+    em("const int tab_a = q.bra_swap ? 1 : N, tab_b = q.bra_swap ? N : 1;")
+    em("const int xba = q.bra_swap ? q.a0 * N + q.b0 : q.b0 * N + q.a0;")
+    em("const int xba_a = q.bra_swap ? N : 1, xba_b = q.bra_swap ? 1 : N;")
+    em("const int tcd = q.ket_swap ? q.d0 * N + q.c0 : q.c0 * N + q.d0;")
+    em("const int tcd_c = q.ket_swap ? 1 : N, tcd_d = q.ket_swap ? N : 1;")
+    em("const int xdc = q.ket_swap ? q.c0 * N + q.d0 : q.d0 * N + q.c0;")
+    em("const int xdc_c = q.ket_swap ? N : 1, xdc_d = q.ket_swap ? 1 : N;")
+
+
+def _ab(form, ia, ib):
+    return f"{form} + {ia} * {form}_a + {ib} * {form}_b"
+
+
+def _cd(form, ic, idd):
+    return f"{form} + {ic} * {form}_c + {idd} * {form}_d"
+
+
+def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, chunk):
+    """Digest one chunk (fixed c components, all d) for every density slot.
+
+    Coulomb (forward, canonical order): G(1,2) += X(4,3) v  [persistent over chunks],
+    G(3,4) += X(2,1) v.  Exchange (forward, kernel order):
+    G(a,c) += X(b,d) v/2, G(b,d) += X(a,c) v/2, G(a,d) += X(b,c) v/2, G(b,c) += X(a,d) v/2.
+    A non-symmetric density adds the same updates taken with X^T into GJ2 / GK2, which the
+    finalisation adds transposed (the reference's remaining four / two updates)."""
+    em("#pragma unroll")
+    em("for (int s = 0; s < NSLOT; ++s) {")
+    em.ind += 1
+    em("const double* __restrict__ D = jk.D[s];")
+    _jk_addressing(em)
+    em("const double dg = q.deg;")
+    em("const double hk = 0.5 * q.deg;")
+    em("const bool ns = jk.nonsym[s] != 0;")
+    em("if (jk.do_j[s]) {")
+    em.ind += 1
+    for ic in chunk:
+        for idd in range(ND):
+            em(f"{{ const double x43 = D[{_cd('xdc', ic, idd)}];")
+            em("  double jcd = 0.0;")
+            for ia in range(NA):
+                for ib in range(NB):
+                    em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('xba', ia, ib)}];")
+                    em(f"  jab_{ia}_{ib}[s] += o_{ia}_{ib}_{ic}_{idd} * x43;")
+            em(f"  atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
+    em("if (ns) {")
+    em.ind += 1
+    for ic in chunk:
+        for idd in range(ND):
+            em(f"{{ const double x34 = dg * D[{_cd('tcd', ic, idd)}];")
+            em("  double jcd = 0.0;")
+            for ia in range(NA):
+                for ib in range(NB):
+                    em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('tab', ia, ib)}];")
+            for ia in range(NA):
+                for ib in range(NB):
+                    em(f"  atomicAdd(&jk.GJ2[s][{_ab('tab', ia, ib)}], o_{ia}_{ib}_{ic}_{idd} * x34);")
+            em(f"  atomicAdd(&jk.GJ2[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
+    em.ind -= 1
+    em("}")
+    em.ind -= 1
+    em("}")
+    em("if (jk.do_k[s]) {")
+    em.ind += 1
+    em("double* __restrict__ GK = jk.GK[s];")
+
+    def kblock(G, trans):
+        # D element (r, c) in kernel order; trans reads X^T
+        def X(r0, ri, c0, ci):
+            if trans:
+                return f"D[(q.{c0} + {ci}) * N + q.{r0} + {ri}]"
+            return f"D[(q.{r0} + {ri}) * N + q.{c0} + {ci}]"
+        for ia in range(NA):
+            for ic in chunk:
+                em("{ double acc = 0.0;")
+                for ib in range(NB):
+                    for idd in range(ND):
+                        em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'd0', idd)};")
+                em(f"  atomicAdd(&{G}[(q.a0 + {ia}) * N + q.c0 + {ic}], hk * acc); }}")
+        for ib in range(NB):
+            for ic in chunk:
+                em("{ double acc = 0.0;")
+                for ia in range(NA):
+                    for idd in range(ND):
+                        em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'd0', idd)};")
+                em(f"  atomicAdd(&{G}[(q.b0 + {ib}) * N + q.c0 + {ic}], hk * acc); }}")
+        return X
+
+    X = kblock("GK", False)
+    # persistent: K(a,d) += X(b,c) ; K(b,d) += X(a,c)
+    for ic in chunk:
+        for ib in range(NB):
+            em(f"{{ const double xbc = {X('b0', ib, 'c0', ic)};")
+            for ia in range(NA):
+                for idd in range(ND):
+                    em(f"  kad_{ia}_{idd}[s] += o_{ia}_{ib}_{ic}_{idd} * xbc;")
+            em("}")
+        for ia in range(NA):
+            em(f"{{ const double xac = {X('a0', ia, 'c0', ic)};")
+            for ib in range(NB):
+                for idd in range(ND):
+                    em(f"  kbd_{ib}_{idd}[s] += o_{ia}_{ib}_{ic}_{idd} * xac;")
+            em("}")
+    em("if (ns) {")
+    em.ind += 1
+    em("double* __restrict__ GK2 = jk.GK2[s];")
+    XT = kblock("GK2", True)
+    for ia in range(NA):
+        for idd in range(ND):
+            em("{ double acc = 0.0;")
+            for ib in range(NB):
+                for ic in chunk:
+                    em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {XT('b0', ib, 'c0', ic)};")
+            em(f"  atomicAdd(&GK2[(q.a0 + {ia}) * N + q.d0 + {idd}], hk * acc); }}")
+    for ib in range(NB):
+        for idd in range(ND):
+            em("{ double acc = 0.0;")
+            for ia in range(NA):
+                for ic in chunk:
+                    em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {XT('a0', ia, 'c0', ic)};")
+            em(f"  atomicAdd(&GK2[(q.b0 + {ib}) * N + q.d0 + {idd}], hk * acc); }}")
+    em.ind -= 1
+    em("}")
+    em.ind -= 1
+    em("}")
+    em.ind -= 1
+    em("}")
+
+
+def gen_digest_final(em: Emitter, NA, NB, ND):
+    em("#pragma unroll")
+    em("for (int s = 0; s < NSLOT; ++s) {")
+    em.ind += 1
+    em("const int N = jk.ncart;")
+    em("const int tab = q.bra_swap ? q.b0 * N + q.a0 : q.a0 * N + q.b0;")
+    em("const int tab_a = q.bra_swap ? 1 : N, tab_b = q.bra_swap ? N : 1;")
+    em("const double dg = q.deg;")
+    em("const double hk = 0.5 * q.deg;")
+    em("if (jk.do_j[s]) {")
+    for ia in range(NA):
+        for ib in range(NB):
+            em(f"  jab_add(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}[s], q.bra_uniform);")
+    em("}")
+    em("if (jk.do_k[s]) {")
+    for ia in range(NA):
+        for idd in range(ND):
+            em(f"  atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
+    for ib in range(NB):
+        for idd in range(ND):
+            em(f"  atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], hk * kbd_{ib}_{idd}[s]);")
+    em("}")
+    em.ind -= 1
+    em("}")
+
+
+def class_list(maxl=2):
+    pairs = []
+    for la in range(maxl + 1):
+        for lb in range(la + 1):
+            pairs.append((la, lb))
+    return pairs
+
+
+def orientation(X, Y):
+    """Choose which pair class is the bra: the one giving the smaller W array
+    (bra Hermite count x ket Cartesian count)."""
+    (la, lb), (lc, ld) = X, Y
+    w1 = len(hermite_list(la + lb)) * ncart(lc) * ncart(ld)
+    w2 = len(hermite_list(lc + ld)) * ncart(la) * ncart(lb)
+    return (X, Y) if w1 <= w2 else (Y, X)
+
+
+def main(out_dir: str, maxl: int = 2):
+    """Write one translation unit per class pair plus the registry include."""
+    import os
+    os.makedirs(out_dir, exist_ok=True)
+    pcl = class_list(maxl)
+    table = []
+    for i, X in enumerate(pcl):
+        for j, Y in enumerate(pcl):
+            if j > i:
+                continue
+            bra, ket = orientation(X, Y)
+            LA, LB = bra
+            LC, LD = ket
+            name = f"{CLS_NAME[LA]}{CLS_NAME[LB]}{CLS_NAME[LC]}{CLS_NAME[LD]}"
+            L = LA + LB + LC + LD
+            src = ["// GENERATED by gen_eri.py -- do not edit.",
+                   '#include "../gdirect_kernels.h"', ""]
+            src.append(gen_class(LA, LB, LC, LD, "jk"))
+            src.append(gen_class(LA, LB, LC, LD, "block"))
+            src.append(f"DEFINE_CLASS_KERNELS({i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {name}, {L})")
+            src.append(f"void cq_register_{name}(ClassKernels& ck) {{")
+            src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, \"{name}\", "
+                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}}};")
+            src.append("}")
+            with open(os.path.join(out_dir, f"eri_{name}.hip"), "w") as f:
+                f.write("\n".join(src) + "\n")
+            table.append(name)
+    reg = ["// GENERATED by gen_eri.py -- do not edit."]
+    for name in table:
+        reg.append(f"void cq_register_{name}(ClassKernels& ck);")
+    reg.append(f"#define CQ_NUM_CLASS_PAIRS {len(table)}")
+    reg.append("static inline void cq_register_all(ClassKernels* t) {")
+    for k, name in enumerate(table):
+        reg.append(f"  cq_register_{name}(t[{k}]);")
+    reg.append("}")
+    with open(os.path.join(out_dir, "eri_registry.inc"), "w") as f:
+        f.write("\n".join(reg) + "\n")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1], int(sys.argv[2]) if len(sys.argv) > 2 else 2)

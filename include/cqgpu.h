@@ -1,0 +1,98 @@
+/* cqgpu.h -- C ABI of the MI355X direct-SCF two-electron contraction G[D].
+ *
+ * Replaces, behind ChronusQ's own operator API, the body of
+ *   template<typename Op, typename T>
+ *   void AOIntegrals::newTwoEContractDirect(const std::vector<std::reference_wrapper<Op>>& X,
+ *        std::vector<std::reference_wrapper<Op>>& AX,
+ *        std::vector<ERI_CONTRACTION_TYPE>& contractionList, std::vector<T>& scalingFactors)
+ * (include/aointegrals/aointegrals_contract.h:24-194, declared include/aointegrals.h:474-480),
+ * together with the state it reads: the shell list (BasisSet::shells, mapSh2Bf, nShell;
+ * include/basisset.h), the Schwarz bounds (AOIntegrals::computeSchwartz,
+ * src/aointegrals/aointegrals_builders.cpp:469-518) and the screening threshold
+ * (AOIntegrals::thresholdSchwartz_, include/aointegrals.h:334).
+ *
+ * Conventions (all as in the reference):
+ *   - matrices are column-major N x N with leading dimension ld (Eigen default);
+ *   - N counts spherical functions for pure shells (2l+1, m = -l..l) and Cartesian
+ *     functions otherwise (x^l ... z^l, each carrying the x^l normalisation);
+ *   - AX accumulates; callers zero it first (include/singleslater/singleslater_fock.h:47-67);
+ *   - COULOMB adds 2*scale*J[X], EXCHANGE adds 2*scale*K[X]; scale == 0 matrices are not
+ *     contracted but still enter the screening bound.
+ * Errors are return codes (never exceptions); cq_last_error() gives the message.  Calls on one
+ * context are synchronous (cq_contract_direct) and not thread-safe.
+ */
+#ifndef CQGPU_H
+#define CQGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cq_ctx cq_ctx_t;
+
+/* return codes */
+#define CQ_OK 0
+#define CQ_ERR_INVALID -1   /* bad argument (reference: assert / CErr) */
+#define CQ_ERR_NOMEM -2     /* device allocation failed (reference: std::bad_alloc) */
+#define CQ_ERR_DEVICE -3    /* HIP runtime error */
+
+/* ERI_CONTRACTION_TYPE (include/aointegrals.h:468-472); PAIR is not handled by the direct
+ * path in the reference either. */
+#define CQ_COULOMB 0
+#define CQ_EXCHANGE 1
+
+/* precision modes */
+#define CQ_PREC_FP64 0
+
+/* Create a context from a flat shell list (replaces the BasisSet/AOIntegrals state that
+ * newTwoEContractDirect reads, aointegrals_contract.h:45-48,96-109).
+ *   l[s], pure[s], nprim[s]: angular momentum (0..2), pure flag, primitive count;
+ *   alpha, coef: exponents and normalised contraction coefficients, concatenated by shell
+ *               (the libint2::Shell coefficients after BasisSet::renormShells);
+ *   origin: 3*nshell centre coordinates in bohr.
+ * Shell order defines function order (mapSh2Bf, src/basisset/basisset_map.cpp:29-40).
+ * Computes the Schwarz bounds on the device (computeSchwartz). */
+int cq_ctx_create(cq_ctx_t** ctx, int nshell, const int* l, const int* pure, const int* nprim,
+                  const double* alpha, const double* coef, const double* origin);
+int cq_ctx_destroy(cq_ctx_t* ctx);
+
+/* Message for the last failing call on ctx (or of cq_ctx_create when ctx is NULL). */
+const char* cq_last_error(const cq_ctx_t* ctx);
+
+/* Schwarz screening threshold tau; the reference hard-codes 1e-14
+ * (include/aointegrals.h:334) and has no setter. */
+int cq_set_schwarz_threshold(cq_ctx_t* ctx, double tau);
+int cq_get_schwarz_threshold(const cq_ctx_t* ctx, double* tau);
+
+/* Precision mode of the quartet arithmetic; only CQ_PREC_FP64 is provided. */
+int cq_set_precision_mode(cq_ctx_t* ctx, int mode, double threshold);
+
+/* Sizes: shells, basis functions, Cartesian functions, shell pairs, primitive pairs kept. */
+int cq_get_dims(const cq_ctx_t* ctx, int* nshell, int* nbf, int* ncart, int* npair, int* nprim_pair);
+
+/* Q(s1,s2) = sqrt(max |(ij|ij)|), nshell x nshell symmetric (AOIntegrals::schwartz_). */
+int cq_get_schwarz(const cq_ctx_t* ctx, double* Q);
+
+/* newTwoEContractDirect on host matrices: X[i], AX[i] column-major n x n, leading
+ * dimension ld; AX[i] += 1/2 scale[i] G_i (aointegrals_contract.h:189-193).  Synchronous. */
+int cq_contract_direct(cq_ctx_t* ctx, int nmat, const double* const* X, double* const* AX,
+                       const int* type, const double* scale, int n, int ld);
+
+/* The same contraction on device-resident matrices, enqueued on stream (a hipStream_t;
+ * NULL selects the context's stream).  Returns once the work is enqueued. */
+int cq_contract_direct_device(cq_ctx_t* ctx, int nmat, const double* const* X, double* const* AX,
+                              const int* type, const double* scale, int n, int ld, void* stream);
+
+/* Surviving quartets per class pair of the last contraction (synchronises). */
+int cq_get_quartet_counts(cq_ctx_t* ctx, long long* counts, int max_counts, int* n_out);
+int cq_num_class_pairs(void);
+int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld);
+
+/* Contracted quartets (s1 s2|s3 s4) in the shells' own basis, row-major per quartet,
+ * concatenated (the buffer libint2::Engine::compute returns, aointegrals_contract.h:138-143). */
+int cq_debug_quartets(cq_ctx_t* ctx, int nq, const int* shells4, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CQGPU_H */

@@ -60,6 +60,8 @@ def lib():
         L.orc_contract_direct.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_int,
                                           P(dp), P(dp), ip, dp, ctypes.c_int]
         L.orc_eri_tensor.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
+        L.orc_count_screened.restype = ctypes.c_long
+        L.orc_count_screened.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
         L.orc_one_electron.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
@@ -140,6 +142,14 @@ class OracleBasis:
         t = np.ascontiguousarray(types, dtype=np.int32)
         s = np.ascontiguousarray(scales, dtype=np.float64)
         return int(lib().orc_contract_direct(self.h, _dp(Qf), float(tau), n, xp, ap, _ip(t), _dp(s), nthreads))
+
+    def count_screened(self, Q: np.ndarray, tau: float, X: Sequence[np.ndarray]) -> int:
+        """Surviving canonical quartets for densities X (screening only)."""
+        Nm = np.zeros((self.nshell, self.nshell), order="F")
+        for x in X:
+            Nm = np.maximum(Nm, self.shblk_norm(x))
+        Qf = np.asfortranarray(Q, dtype=np.float64)
+        return int(lib().orc_count_screened(self.nshell, _dp(Nm), _dp(Qf), float(tau)))
 
     def eri_tensor(self, nthreads: int = 8) -> np.ndarray:
         N = self.nbf

@@ -766,4 +766,26 @@ void orc_one_electron(void* h, int natom, const double* Z, const double* xyz, do
   overlap_kinetic_nuclear(*static_cast<Basis*>(h), natom, Z, xyz, S, T, V);
 }
 
+// Screening decisions only (the test of aointegrals_contract.h:111-131): number of canonical
+// quartets that survive, for a combined norm matrix Nmax (nshell^2, column-major) and Q.
+long orc_count_screened(int ns, const double* Nmax, const double* Q, double tau) {
+  auto nrm = [&](int a, int b) { return Nmax[(size_t)b * ns + a]; };
+  auto q = [&](int a, int b) { return Q[(size_t)b * ns + a]; };
+  long n = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : n)
+  for (int s1 = 0; s1 < ns; ++s1)
+    for (int s2 = 0; s2 <= s1; ++s2)
+      for (int s3 = 0; s3 <= s1; ++s3) {
+        const int s4max = (s1 == s3) ? s2 : s3;
+        for (int s4 = 0; s4 <= s4max; ++s4) {
+          const double shMax = std::max(nrm(s1, s4), std::max(nrm(s2, s4), std::max(nrm(s3, s4),
+                               std::max(nrm(s1, s3), std::max(nrm(s2, s3), nrm(s1, s2)))))) *
+                               q(s1, s2) * q(s3, s4);
+          if (!(shMax < tau)) ++n;
+        }
+      }
+  return n;
+}
+
 }  // extern "C"
+

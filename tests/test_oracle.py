@@ -1,0 +1,231 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Every oracle function is checked against something independent of its own formula:
+golden vectors (scipy Boys values, the reference's golden SCF energies), closed forms,
+a second algorithm (long-double McMurchie-Davidson vs Obara-Saika/HGP), the derivative
+identity for angular momentum, permutational symmetry, and brute-force N^4 contraction.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import scf
+from chronusq_public_amd import basis as B
+from chronusq_public_amd import molecule as M
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def water():
+    return M.water_test0001()
+
+
+def _bs(name, mol):
+    return B.build_basis(name, mol.Z, mol.coords_bohr)
+
+
+def test_boys_against_scipy_golden():
+    d = json.load(open(os.path.join(GOLD, "boys.json")))
+    worst = 0.0
+    for row in d["rows"]:
+        F = oracle.boys(16, row["T"])
+        ref = np.array(row["F"])
+        rel = np.abs(F - ref) / np.maximum(np.abs(ref), 1e-300)
+        worst = max(worst, rel.max())
+    assert worst < 5e-13, worst
+
+
+def test_boys_closed_forms():
+    F = oracle.boys(10, 0.0)
+    assert np.allclose(F, [1.0 / (2 * m + 1) for m in range(11)], rtol=0, atol=1e-16)
+    T = 2.3
+    F = oracle.boys(1, T)
+    assert abs(F[0] - 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))) < 1e-15
+    # recursion identity F_m = (2T F_{m+1} + e^-T) / (2m+1)
+    F = oracle.boys(12, 7.7)
+    for m in range(12):
+        assert abs(F[m] - (2 * 7.7 * F[m + 1] + math.exp(-7.7)) / (2 * m + 1)) < 1e-15
+
+
+def test_car2sph_d_closed_form():
+    s3 = math.sqrt(3.0)
+    ref = np.array([[0, s3, 0, 0, 0, 0],
+                    [0, 0, 0, 0, s3, 0],
+                    [-0.5, 0, 0, -0.5, 0, 1.0],
+                    [0, 0, s3, 0, 0, 0],
+                    [s3 / 2, 0, 0, -s3 / 2, 0, 0]])
+    assert np.abs(oracle.car2sph(2) - ref).max() < 1e-14
+
+
+def test_basis_function_counts_and_normalisation(water):
+    # reference rules: SP split, pure d (basisset_read.cpp:240-255)
+    assert _bs("6-31g*", water).nbasis == 18
+    assert _bs("cc-pvdz", water).nbasis == 24
+    assert _bs("sto-3g", water).nbasis == 7
+    for name in ("sto-3g", "6-31g", "6-31g*", "cc-pvdz"):
+        ob = oracle.OracleBasis(_bs(name, water))
+        S, T, V = ob.one_electron(water)
+        assert np.abs(np.diag(S) - 1.0).max() < 1e-13   # renormShells
+        assert np.abs(S - S.T).max() < 1e-14
+
+
+def test_ssss_closed_form(water):
+    bs = _bs("sto-3g", water)
+    ob = oracle.OracleBasis(bs)
+    sh = bs.shells
+    # (s_O1 s_H | s_H s_O2) closed form by explicit primitive sums
+    idx = [0, 3, 4, 1]
+    A, Bs, C, D = (sh[i] for i in idx)
+    ref = 0.0
+    for a, ca in zip(A.alpha, A.coef):
+        for b, cb in zip(Bs.alpha, Bs.coef):
+            p = a + b
+            P = (a * A.origin + b * Bs.origin) / p
+            Kab = math.exp(-a * b / p * np.sum((A.origin - Bs.origin) ** 2))
+            for c, cc in zip(C.alpha, C.coef):
+                for d, cd in zip(D.alpha, D.coef):
+                    q = c + d
+                    Q = (c * C.origin + d * D.origin) / q
+                    Kcd = math.exp(-c * d / q * np.sum((C.origin - D.origin) ** 2))
+                    T = p * q / (p + q) * np.sum((P - Q) ** 2)
+                    F0 = 1.0 if T == 0 else 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+                    ref += ca * cb * cc * cd * 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * Kab * Kcd * F0
+    for algo in (0, 1, 2):
+        assert abs(ob.quartet(*idx, algo=algo)[0, 0, 0, 0] - ref) < 1e-14
+
+
+def test_os_vs_md_all_quartets(water):
+    """Obara-Saika/HGP (double) against McMurchie-Davidson (long double), every quartet class."""
+    bs = _bs("cc-pvdz", water)
+    ob = oracle.OracleBasis(bs)
+    ns = bs.nshell
+    worst = 0.0
+    for s1 in range(ns):
+        for s2 in range(s1 + 1):
+            for s3 in range(s1 + 1):
+                for s4 in range(s3 + 1):
+                    if (s1 * 31 + s2 * 7 + s3 * 3 + s4) % 3:
+                        continue
+                    a = ob.quartet(s1, s2, s3, s4, 0)
+                    b = ob.quartet(s1, s2, s3, s4, 2)
+                    worst = max(worst, np.abs(a - b).max())
+    assert worst < 1e-13, worst
+
+
+def test_derivative_identity_p_from_s():
+    """(p_x s|s s) = 1/(2 alpha) d/dA_x (s s|s s) for one primitive (finite differences)."""
+    a, b, c, d = 0.8, 0.5, 1.1, 0.3
+    O = np.array([[0.1, -0.2, 0.3], [0.7, 0.2, -0.4], [-0.5, 0.6, 0.2], [0.3, -0.3, 0.9]])
+
+    def quartet(la, origins):
+        shells = []
+        for l, al, o in zip([la, 0, 0, 0], [a, b, c, d], origins):
+            shells.append(B.Shell(l=l, pure=False, alpha=np.array([al]), coef=np.array([1.0]),
+                                  origin=o.copy(), atom=0))
+        bs = B.BasisSet("x", shells)
+        return oracle.OracleBasis(bs).quartet(0, 1, 2, 3, algo=1)
+
+    pss = quartet(1, O)[:, 0, 0, 0]
+    h = 1e-5
+    for k in range(3):
+        Op, Om = O.copy(), O.copy()
+        Op[0, k] += h
+        Om[0, k] -= h
+        dv = (quartet(0, Op)[0, 0, 0, 0] - quartet(0, Om)[0, 0, 0, 0]) / (2 * h)
+        assert abs(pss[k] - dv / (2 * a)) < 1e-9
+
+
+def test_permutational_symmetry(water):
+    bs = _bs("6-31g*", water)
+    ob = oracle.OracleBasis(bs)
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        s = rng.integers(0, bs.nshell, size=4)
+        v = ob.quartet(*s)
+        assert np.abs(v - ob.quartet(s[1], s[0], s[2], s[3]).transpose(1, 0, 2, 3)).max() < 1e-14
+        assert np.abs(v - ob.quartet(s[2], s[3], s[0], s[1]).transpose(2, 3, 0, 1)).max() < 1e-14
+        assert np.abs(v - ob.quartet(s[0], s[1], s[3], s[2]).transpose(0, 1, 3, 2)).max() < 1e-14
+
+
+def test_schwarz_definition_and_inequality(water):
+    bs = _bs("6-31g*", water)
+    ob = oracle.OracleBasis(bs)
+    Q = ob.schwarz()
+    assert np.abs(Q - Q.T).max() == 0.0
+    ns = bs.nshell
+    for s1 in range(ns):
+        for s2 in range(s1 + 1):
+            v = ob.quartet(s1, s2, s1, s2)
+            n1, n2 = v.shape[0], v.shape[1]
+            diag = max(abs(v[i, j, i, j]) for i in range(n1) for j in range(n2))
+            assert Q[s1, s2] == math.sqrt(diag)
+            for s3 in range(ns):
+                for s4 in range(s3 + 1):
+                    w = ob.quartet(s1, s2, s3, s4)
+                    assert np.abs(w).max() <= Q[s1, s2] * Q[s3, s4] * (1 + 1e-12)
+
+
+@pytest.mark.parametrize("name", ["6-31g*", "cc-pvdz"])
+def test_direct_matches_brute_force(water, name):
+    """Direct restatement vs the N^4 in-core formulas (aointegrals_contract.h:239-259).
+
+    Symmetric X: J and K agree element-wise.  Non-symmetric X: K agrees; the reference's
+    direct Coulomb scatter uses X(4,3)/X(2,1) in canonical shell order, so only its
+    symmetric part equals the in-core J, and J[X^T] = J[X]^T."""
+    bs = _bs(name, water)
+    ob = oracle.OracleBasis(bs)
+    N = bs.nbasis
+    eri = ob.eri_tensor()
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-1, 1, (N, N))
+    X = np.asfortranarray(X + X.T)
+    Xn = np.asfortranarray(rng.uniform(-1, 1, (N, N)))   # non-symmetric
+    Q = ob.schwarz()
+
+    def direct(Xi):
+        J = np.zeros((N, N), order="F")
+        K = np.zeros((N, N), order="F")
+        ob.contract_direct(Q, 0.0, [Xi, Xi], [J, K], [oracle.COULOMB, oracle.EXCHANGE], [1.0, -0.5])
+        return J, K
+
+    J, K = direct(X)
+    assert np.abs(J - oracle.brute_force_contract(eri, X, oracle.COULOMB, 1.0)).max() < 1e-12
+    assert np.abs(K - oracle.brute_force_contract(eri, X, oracle.EXCHANGE, -0.5)).max() < 1e-12
+    assert np.abs(J - J.T).max() < 1e-12 and np.abs(K - K.T).max() < 1e-12
+    assert np.sum(X * J) > 0.0
+    Jn, Kn = direct(Xn)
+    Jb = oracle.brute_force_contract(eri, Xn, oracle.COULOMB, 1.0)
+    assert np.abs(Kn - oracle.brute_force_contract(eri, Xn, oracle.EXCHANGE, -0.5)).max() < 1e-12
+    assert np.abs(0.5 * (Jn + Jn.T) - Jb).max() < 1e-12
+    JnT, KnT = direct(np.asfortranarray(Xn.T))
+    assert np.abs(JnT - Jn.T).max() < 1e-12 and np.abs(KnT - Kn.T).max() < 1e-12
+
+
+def test_thread_count_invariance(water):
+    bs = _bs("6-31g*", water)
+    ob = oracle.OracleBasis(bs)
+    N = bs.nbasis
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (N, N))
+    X = np.asfortranarray(X + X.T)
+    Q = ob.schwarz()
+    outs = []
+    for nt in (1, 3, 8):
+        J = np.zeros((N, N), order="F")
+        n = ob.contract_direct(Q, 1e-12, [X], [J], [0], [1.0], nthreads=nt)
+        outs.append((J, n))
+    for J, n in outs[1:]:
+        assert np.abs(J - outs[0][0]).max() < 1e-13 and n == outs[0][1]
+
+
+def test_golden_scf_energies(water):
+    d = json.load(open(os.path.join(GOLD, "energies.json")))
+    for test, rec in d["tests"].items():
+        r = scf.rhf(water, _bs(rec["basis"], water), tau=1e-14)
+        assert r["converged"]
+        assert abs(r["E"] - rec["E"]) < d["tolerance"], (test, r["E"], rec["E"])
