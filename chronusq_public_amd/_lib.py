@@ -24,7 +24,8 @@ EXPORTED = [
     "cq_ctx_create", "cq_ctx_destroy", "cq_last_error", "cq_set_schwarz_threshold",
     "cq_get_schwarz_threshold", "cq_set_precision_mode", "cq_get_dims", "cq_get_schwarz",
     "cq_contract_direct", "cq_contract_direct_device", "cq_get_quartet_counts",
-    "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets",
+    "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
+    "cq_get_kernel_times", "cq_get_work_stats",
 ]
 
 _lib = None
@@ -67,6 +68,9 @@ def load():
     L.cq_num_class_pairs.argtypes = []
     L.cq_class_pair_info.argtypes = [ctypes.c_int, ip, ip, ip, ip]
     L.cq_debug_quartets.argtypes = [vp, ctypes.c_int, ip, dp]
+    L.cq_set_profiling.argtypes = [vp, ctypes.c_int]
+    L.cq_get_kernel_times.argtypes = [vp, dp, P(ctypes.c_longlong), ctypes.c_int]
+    L.cq_get_work_stats.argtypes = [vp, dp, ctypes.c_int]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int or name not in ("cq_last_error",):

@@ -42,16 +42,17 @@ def build(incremental: bool = False, resource_usage: bool = False, jobs: int = 0
     gen_script = os.path.join(CSRC, "gen_eri.py")
     subprocess.run([sys.executable, gen_script, GEN], check=True)
     srcs = sorted(glob.glob(os.path.join(GEN, "*.hip"))) + [os.path.join(CSRC, "gdirect.hip")]
-    deps = [os.path.join(CSRC, f) for f in ("gdirect_device.h", "gdirect_kernels.h")] + \
-        [os.path.join(HERE, "..", "include", "cqgpu.h"), gen_script]
+    deps = [os.path.join(CSRC, f) for f in ("gdirect_device.h", "gdirect_kernels.h")] + [gen_script]
     dep_mtime = max(os.path.getmtime(d) for d in deps)
+    header_mtime = os.path.getmtime(os.path.join(HERE, "..", "include", "cqgpu.h"))
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
     todo = []
     objs = []
     for s in srcs:
         o = os.path.join(BUILD, os.path.basename(s) + ".o")
         objs.append(o)
-        if incremental and os.path.exists(o) and os.path.getmtime(o) > max(os.path.getmtime(s), dep_mtime):
+        newest = max(os.path.getmtime(s), dep_mtime, header_mtime if s.endswith("gdirect.hip") else 0.0)
+        if incremental and os.path.exists(o) and os.path.getmtime(o) > newest:
             continue
         todo.append((s, o))
     jobs = jobs or min(len(todo) or 1, os.cpu_count() or 4, 16)

@@ -250,6 +250,47 @@ __global__ void schwarz_kernel(const double* __restrict__ blocks, const int* __r
   pairQ[P] = sqrt(mx);
 }
 
+// Work sums over one class pair's task list, for the algorithmic FLOP model:
+// acc[0] += K_ab K_cd (primitive quartets), acc[1] += K_ab, acc[2] += sequential
+// Cartesian->spherical transform cost, acc[3] += output integrals in the shells' basis.
+__global__ void work_stats_kernel(const uint2* __restrict__ tasks, const unsigned long long* __restrict__ count_ptr,
+                                  const PairInfo* __restrict__ pairs, const ShellMeta* __restrict__ sh,
+                                  double* __restrict__ acc) {
+  const long count = (long)*count_ptr;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
+    const uint2 tk = tasks[t];
+    const PairInfo B = pairs[tk.x], K = pairs[tk.y];
+    const double kab = B.p1 - B.p0, kcd = K.p1 - K.p0;
+    s0 += kab * kcd;
+    s1 += kab;
+    const int shl[4] = {B.a, B.b, K.a, K.b};
+    double dims[4];
+    int l4[4], p4[4];
+    for (int k = 0; k < 4; ++k) {
+      const ShellMeta m = sh[shl[k]];
+      l4[k] = m.l; p4[k] = m.pure;
+      dims[k] = (m.l + 1) * (m.l + 2) / 2;
+    }
+    double cost = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      if (!p4[k] || l4[k] < 2) continue;
+      double other = 1.0;
+      for (int q = 0; q < 4; ++q) if (q != k) other *= dims[q];
+      cost += 2.0 * (2 * l4[k] + 1) * dims[k] * other;
+      dims[k] = 2 * l4[k] + 1;
+    }
+    s2 += cost;
+    double nsph = 1.0;
+    for (int k = 0; k < 4; ++k) nsph *= p4[k] ? (2 * l4[k] + 1) : (l4[k] + 1) * (l4[k] + 2) / 2;
+    s3 += nsph;
+  }
+  atomicAdd(&acc[0], s0);
+  atomicAdd(&acc[1], s1);
+  atomicAdd(&acc[2], s2);
+  atomicAdd(&acc[3], s3);
+}
+
 // =======================================================================================
 // Host side
 // =======================================================================================
@@ -384,8 +425,24 @@ struct cq_ctx {
   int nitem = 0;
   long long task_capacity = 0;
   std::vector<unsigned long long> last_counts;
+  // profiling: events around every class-pair kernel of every contraction
+  bool profiling = false;
+  std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof_events;
+  std::vector<double> prof_ms;
+  std::vector<long long> prof_launches;
+  std::vector<hipEvent_t> event_pool;
+  size_t event_next = 0;
+  hipEvent_t get_event() {
+    if (event_next == event_pool.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      event_pool.push_back(e);
+    }
+    return event_pool[event_next++];
+  }
   std::vector<double> tco;
   std::vector<int> tco_off;
+  std::vector<double> boys_host;
   // device
   DevBuf<ShellMeta> d_shells;
   DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start, d_flags;
@@ -402,6 +459,7 @@ struct cq_ctx {
     d_item_start.release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
     d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
     d_cps.release(); d_tasks.release(); d_counts.release(); d_nglob.release();
+    for (auto e : event_pool) (void)hipEventDestroy(e);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -516,7 +574,8 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_sph2shell.upload(c->sph2shell, c->stream);
   c->d_tco.upload(c->tco, c->stream);
   c->d_tco_off.upload(c->tco_off, c->stream);
-  c->d_boys.upload(boys_table(), c->stream);
+  c->boys_host = boys_table();   // stays alive until the stream is synchronised
+  c->d_boys.upload(c->boys_host, c->stream);
   c->d_pairs.upload(c->pairs, c->stream);
   c->d_prims.upload(c->prims, c->stream);
   c->d_pairQ.ensure(npair);
@@ -605,6 +664,9 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_nglob.ensure(1);
   c->d_Nmax.ensure((size_t)nshell * nshell);
   c->last_counts.assign(kNumClassPairs, 0);
+  c->prof_events.assign(kNumClassPairs, {});
+  c->prof_ms.assign(kNumClassPairs, 0.0);
+  c->prof_launches.assign(kNumClassPairs, 0);
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
@@ -699,9 +761,19 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
                                            : (long long)c->cps[k].nbra * c->cps[k].nket;
       const int grid = (int)std::min<long long>(persistent_grid(), (cap + WG - 1) / WG);
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (c->profiling) {
+        e0 = c->get_event();
+        e1 = c->get_event();
+        HIPCHK(hipEventRecord(e0, s));
+      }
       hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, s, tasks, c->d_counts.p + k, c->d_pairs.p,
                          c->d_prims.p, c->d_boys.p, jk);
       HIPCHK(hipGetLastError());
+      if (c->profiling) {
+        HIPCHK(hipEventRecord(e1, s));
+        c->prof_events[k].push_back({e0, e1});
+      }
     }
   }
   // 5. back to the shells' basis and accumulate: AX += 1/2 scale (G + G^T-partner)
@@ -881,6 +953,62 @@ int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld_) {
 }
 
 int cq_num_class_pairs(void) { return kNumClassPairs; }
+
+// Per-class-pair kernel timing with HIP events recorded on the launch stream.
+int cq_set_profiling(cq_ctx_t* c, int on) {
+  return guarded(c, [&] {
+    c->profiling = on != 0;
+    for (auto& v : c->prof_events) v.clear();
+    std::fill(c->prof_ms.begin(), c->prof_ms.end(), 0.0);
+    std::fill(c->prof_launches.begin(), c->prof_launches.end(), 0);
+    c->event_next = 0;
+  });
+}
+
+// Accumulated kernel milliseconds and launch counts per class pair since profiling was
+// switched on (synchronises every recorded event).
+int cq_get_kernel_times(cq_ctx_t* c, double* ms, long long* launches, int max_n) {
+  return guarded(c, [&] {
+    for (int k = 0; k < kNumClassPairs; ++k) {
+      for (auto& pr : c->prof_events[k]) {
+        HIPCHK(hipEventSynchronize(pr.second));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, pr.first, pr.second));
+        c->prof_ms[k] += t;
+        c->prof_launches[k] += 1;
+      }
+      c->prof_events[k].clear();
+    }
+    c->event_next = 0;
+    for (int k = 0; k < kNumClassPairs && k < max_n; ++k) {
+      ms[k] = c->prof_ms[k];
+      launches[k] = c->prof_launches[k];
+    }
+  });
+}
+
+// Work sums of the last contraction per class pair (4 doubles each): primitive quartets
+// sum K_ab K_cd, sum K_ab, Cartesian->spherical transform FLOPs, output integrals.
+int cq_get_work_stats(cq_ctx_t* c, double* out, int max_n) {
+  return guarded(c, [&] {
+    DevBuf<double> acc;
+    acc.ensure((size_t)4 * kNumClassPairs);
+    HIPCHK(hipMemsetAsync(acc.p, 0, 4 * kNumClassPairs * sizeof(double), c->stream));
+    for (int k = 0; k < kNumClassPairs; ++k) {
+      if (c->cps[k].nbra == 0 || c->cps[k].nket == 0) continue;
+      hipLaunchKernelGGL(work_stats_kernel, dim3(persistent_grid()), dim3(WG), 0, c->stream,
+                         c->d_tasks.p + c->cps[k].task_off, c->d_counts.p + k, c->d_pairs.p, c->d_shells.p,
+                         acc.p + 4 * k);
+      HIPCHK(hipGetLastError());
+    }
+    std::vector<double> h((size_t)4 * kNumClassPairs);
+    HIPCHK(hipMemcpyAsync(h.data(), acc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    acc.release();
+    for (int k = 0; k < kNumClassPairs && k < max_n; ++k)
+      for (int q = 0; q < 4; ++q) out[4 * k + q] = h[4 * k + q];
+  });
+}
 
 // Contracted quartets (s1 s2|s3 s4) in the shells' own basis, row-major per quartet,
 // concatenated; for per-quartet parity tests.

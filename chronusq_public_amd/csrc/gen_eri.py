@@ -161,43 +161,56 @@ class RGen:
 
 
 # --------------------------------------------------------------------------- class core
-def chunk_plan(LA, LB, LC, LD, budget=96):
+REG_SOFT = 125     # live doubles targeted by the chunk-by-c plan (about 250 VGPRs at most)
+REG_HARD = 170     # above this even one c component per chunk spills: use (c,d) chunks
+
+
+def plan_class(LA, LB, LC, LD):
+    """Split the ket Cartesian components (c, d) into chunks processed one after the other.
+
+    Returns (chunks, persist): chunks is a list of lists of (ic, idd); persist says whether
+    J(1,2), K(a,d), K(b,d) are accumulated in registers across chunks (else every chunk
+    issues its own atomics)."""
     NAB = ncart(LA) * ncart(LB)
     HAB = len(hermite_list(LA + LB))
-    ND = ncart(LD)
-    NC = ncart(LC)
+    NC, ND = ncart(LC), ncart(LD)
     HL = len(hermite_list(LA + LB + LC + LD))
-    persist = NAB + ncart(LA) * ND + ncart(LB) * ND
-    cs = NC
-    while cs > 1 and (NAB + HAB) * cs * ND + HL + persist + 24 > budget:
-        cs -= 1
-    chunks = []
-    c = 0
-    while c < NC:
-        chunks.append(list(range(c, min(NC, c + cs))))
-        c += cs
-    return chunks
+    persist_regs = NAB + ncart(LA) * ND + ncart(LB) * ND
+
+    def regs(nk, persist):
+        return (NAB + HAB) * nk + HL + (persist_regs if persist else 0) + 24
+
+    if regs(ND, True) <= REG_HARD:
+        cs = NC
+        while cs > 1 and regs(cs * ND, True) > REG_SOFT:
+            cs -= 1
+        chunks = []
+        for c0 in range(0, NC, cs):
+            chunks.append([(ic, idd) for ic in range(c0, min(NC, c0 + cs)) for idd in range(ND)])
+        return chunks, True
+    pairs = [(ic, idd) for ic in range(NC) for idd in range(ND)]
+    nk = max(1, (REG_HARD - 20 - HL - 24) // (NAB + HAB))
+    return [pairs[i:i + nk] for i in range(0, len(pairs), nk)], False
 
 
 def gen_class(LA, LB, LC, LD, kind: str) -> str:
-    """kind: 'jk' or 'block'."""
+    """kind: 'jk' (digest into J/K) or 'block' (write the contracted Cartesian block)."""
     name = f"{CLS_NAME[LA]}{CLS_NAME[LB]}{CLS_NAME[LC]}{CLS_NAME[LD]}"
     L = LA + LB + LC + LD
     ca, cb, cc, cd = cart(LA), cart(LB), cart(LC), cart(LD)
     NA, NB, NC, ND = len(ca), len(cb), len(cc), len(cd)
     herm_ab = hermite_list(LA + LB)
-    chunks = chunk_plan(LA, LB, LC, LD) if kind == "jk" else [list(range(NC))]
+    chunks, persist = plan_class(LA, LB, LC, LD)
     em = Emitter()
     em.ind = 0
-    em(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}) kind={kind} chunks={len(chunks)}")
+    em(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}) kind={kind} chunks={len(chunks)} persist={persist}")
     if kind == "jk":
         em("template <int NSLOT>")
         em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds) {{")
     else:
         em(f"__device__ __forceinline__ void quartet_block_{name}(const QuartetArgs& q, double* __restrict__ out, const double* __restrict__ boys_lds) {{")
     em.ind = 1
-    if kind == "jk":
-        # persistent digestion accumulators
+    if kind == "jk" and persist:
         for ia in range(NA):
             for ib in range(NB):
                 em(f"double jab_{ia}_{ib}[NSLOT] = {{}};")
@@ -207,10 +220,9 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
         for ib in range(NB):
             for idd in range(ND):
                 em(f"double kbd_{ib}_{idd}[NSLOT] = {{}};")
-    for ci, chunk in enumerate(chunks):
-        em(f"{{ // chunk {ci}: c components {chunk}")
+    for ci, ks in enumerate(chunks):
+        em(f"{{ // chunk {ci}: (c,d) components {ks}")
         em.ind += 1
-        ks = [(ic, idd) for ic in chunk for idd in range(ND)]
         for ia in range(NA):
             for ib in range(NB):
                 for (ic, idd) in ks:
@@ -219,8 +231,8 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
         em.ind += 1
         em("const PrimPair bp = q.prim[ip];")
         Ebra = {}
-        for d, PAv, PBv in (("x", "bp.PA[0]", "bp.PB[0]"), ("y", "bp.PA[1]", "bp.PB[1]"), ("z", "bp.PA[2]", "bp.PB[2]")):
-            Ebra[d] = gen_E(em, "Eb", d, LA, LB, PAv, PBv, "bp.o2p", sign=False)
+        for d, i in (("x", 0), ("y", 1), ("z", 2)):
+            Ebra[d] = gen_E(em, "Eb", d, LA, LB, f"bp.PA[{i}]", f"bp.PB[{i}]", "bp.o2p", sign=False)
         for (t, u, v) in herm_ab:
             for (ic, idd) in ks:
                 em(f"double W_{t}_{u}_{v}_{ic}_{idd} = 0.0;")
@@ -246,7 +258,6 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
         for d, i in (("x", 0), ("y", 1), ("z", 2)):
             Eket[d] = gen_E(em, "Ek", d, LC, LD, f"kp.PA[{i}]", f"kp.PB[{i}]", "kp.o2p", sign=True)
         rg = RGen(em, L)
-        # C products and W updates per ket component
         for (ic, idd) in ks:
             (kx, ky, kz), (lx, ly, lz) = cc[ic], cd[idd]
             terms = []
@@ -259,30 +270,23 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                     for ph in range(kz + lz + 1):
                         if (kz, lz, ph) not in Eket["z"]:
                             continue
-                        e, s = mul(Eket["x"][(kx, lx, tau)], Eket["y"][(ky, ly, nu)], Eket["z"][(kz, lz, ph)])
+                        e, sg = mul(Eket["x"][(kx, lx, tau)], Eket["y"][(ky, ly, nu)], Eket["z"][(kz, lz, ph)])
                         cname = None
                         if e is not None and "*" in e:
                             cname = f"C_{ic}_{idd}_{tau}_{nu}_{ph}"
-                            em(f"const double {cname} = {'-' if s < 0 else ''}{e};")
-                            s = 1
+                            em(f"const double {cname} = {'-' if sg < 0 else ''}{e};")
+                            sg = 1
                         elif e is not None:
                             cname = e
-                        terms.append((tau, nu, ph, cname, s))
+                        terms.append((tau, nu, ph, cname, sg))
             for (t, u, v) in herm_ab:
-                parts = []
-                for (tau, nu, ph, cname, s) in terms:
-                    r = rg.get(0, t + tau, u + nu, v + ph)
-                    if cname is None:
-                        parts.append(("+" if s > 0 else "-", r))
-                    else:
-                        parts.append(("+" if s > 0 else "-", f"{cname}*{r}"))
                 expr = ""
-                for sg, p in parts:
-                    expr += f" {sg} {p}"
+                for (tau, nu, ph, cname, sg) in terms:
+                    r = rg.get(0, t + tau, u + nu, v + ph)
+                    expr += f" {'+' if sg > 0 else '-'} " + (r if cname is None else f"{cname}*{r}")
                 em(f"W_{t}_{u}_{v}_{ic}_{idd} +={expr};")
         em.ind -= 1
         em("}  // ket primitives")
-        # bra transform
         for ia, (ax, ay, az) in enumerate(ca):
             for ib, (bx, by, bz) in enumerate(cb):
                 bterms = []
@@ -295,7 +299,7 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                         for v in range(az + bz + 1):
                             if (az, bz, v) not in Ebra["z"]:
                                 continue
-                            e, s = mul(Ebra["x"][(ax, bx, t)], Ebra["y"][(ay, by, u)], Ebra["z"][(az, bz, v)])
+                            e, _ = mul(Ebra["x"][(ax, bx, t)], Ebra["y"][(ay, by, u)], Ebra["z"][(az, bz, v)])
                             bn = f"B_{ia}_{ib}_{t}_{u}_{v}"
                             em(f"const double {bn} = bp.K{('*' + e) if e else ''};")
                             bterms.append((t, u, v, bn))
@@ -311,10 +315,10 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                         idx = ((ia * NB + ib) * NC + ic) * ND + idd
                         em(f"out[{idx}] = o_{ia}_{ib}_{ic}_{idd};")
         else:
-            gen_digest_chunk(em, NA, NB, NC, ND, chunk)
+            gen_digest_chunk(em, NA, NB, ks, persist)
         em.ind -= 1
         em("}")
-    if kind == "jk":
+    if kind == "jk" and persist:
         gen_digest_final(em, NA, NB, ND)
     em.ind = 0
     em("}")
@@ -350,14 +354,17 @@ def _cd(form, ic, idd):
     return f"{form} + {ic} * {form}_c + {idd} * {form}_d"
 
 
-def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, chunk):
-    """Digest one chunk (fixed c components, all d) for every density slot.
+def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
+    """Digest one chunk of (c,d) components for every density slot.
 
-    Coulomb (forward, canonical order): G(1,2) += X(4,3) v  [persistent over chunks],
-    G(3,4) += X(2,1) v.  Exchange (forward, kernel order):
-    G(a,c) += X(b,d) v/2, G(b,d) += X(a,c) v/2, G(a,d) += X(b,c) v/2, G(b,c) += X(a,d) v/2.
-    A non-symmetric density adds the same updates taken with X^T into GJ2 / GK2, which the
-    finalisation adds transposed (the reference's remaining four / two updates)."""
+    Coulomb (forward, canonical in-pair order): G(1,2) += X(4,3) v, G(3,4) += X(2,1) v.
+    Exchange (forward, kernel order): G(a,c) += X(b,d) v/2, G(b,d) += X(a,c) v/2,
+    G(a,d) += X(b,c) v/2, G(b,c) += X(a,d) v/2.  A non-symmetric density adds the same
+    updates taken with X^T into GJ2 / GK2, which the finalisation adds transposed (the
+    reference's remaining Coulomb and exchange updates).  With persist, G(1,2), G(a,d) and
+    G(b,d) stay in registers across chunks; otherwise each chunk adds its partial sums."""
+    cs = sorted(set(ic for ic, _ in ks))
+    ds = sorted(set(idd for _, idd in ks))
     em("#pragma unroll")
     em("for (int s = 0; s < NSLOT; ++s) {")
     em.ind += 1
@@ -368,91 +375,105 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, chunk):
     em("const bool ns = jk.nonsym[s] != 0;")
     em("if (jk.do_j[s]) {")
     em.ind += 1
-    for ic in chunk:
-        for idd in range(ND):
-            em(f"{{ const double x43 = D[{_cd('xdc', ic, idd)}];")
-            em("  double jcd = 0.0;")
-            for ia in range(NA):
-                for ib in range(NB):
-                    em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('xba', ia, ib)}];")
-                    em(f"  jab_{ia}_{ib}[s] += o_{ia}_{ib}_{ic}_{idd} * x43;")
-            em(f"  atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
+    if not persist:
+        for ia in range(NA):
+            for ib in range(NB):
+                em(f"double jab_{ia}_{ib} = 0.0;")
+    for (ic, idd) in ks:
+        em(f"{{ const double x43 = D[{_cd('xdc', ic, idd)}];")
+        em("  double jcd = 0.0;")
+        for ia in range(NA):
+            for ib in range(NB):
+                em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('xba', ia, ib)}];")
+                tgt = f"jab_{ia}_{ib}[s]" if persist else f"jab_{ia}_{ib}"
+                em(f"  {tgt} += o_{ia}_{ib}_{ic}_{idd} * x43;")
+        em(f"  atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
+    if not persist:
+        for ia in range(NA):
+            for ib in range(NB):
+                em(f"jab_add(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}, q.bra_uniform);")
     em("if (ns) {")
     em.ind += 1
-    for ic in chunk:
-        for idd in range(ND):
-            em(f"{{ const double x34 = dg * D[{_cd('tcd', ic, idd)}];")
-            em("  double jcd = 0.0;")
-            for ia in range(NA):
-                for ib in range(NB):
-                    em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('tab', ia, ib)}];")
-            for ia in range(NA):
-                for ib in range(NB):
-                    em(f"  atomicAdd(&jk.GJ2[s][{_ab('tab', ia, ib)}], o_{ia}_{ib}_{ic}_{idd} * x34);")
-            em(f"  atomicAdd(&jk.GJ2[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
+    for (ic, idd) in ks:
+        em(f"{{ const double x34 = dg * D[{_cd('tcd', ic, idd)}];")
+        em("  double jcd = 0.0;")
+        for ia in range(NA):
+            for ib in range(NB):
+                em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('tab', ia, ib)}];")
+        for ia in range(NA):
+            for ib in range(NB):
+                em(f"  atomicAdd(&jk.GJ2[s][{_ab('tab', ia, ib)}], o_{ia}_{ib}_{ic}_{idd} * x34);")
+        em(f"  atomicAdd(&jk.GJ2[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
     em.ind -= 1
     em("}")
     em.ind -= 1
     em("}")
     em("if (jk.do_k[s]) {")
     em.ind += 1
-    em("double* __restrict__ GK = jk.GK[s];")
 
-    def kblock(G, trans):
-        # D element (r, c) in kernel order; trans reads X^T
-        def X(r0, ri, c0, ci):
-            if trans:
-                return f"D[(q.{c0} + {ci}) * N + q.{r0} + {ri}]"
-            return f"D[(q.{r0} + {ri}) * N + q.{c0} + {ci}]"
+    def X(r0, ri, c0, ci, trans):
+        if trans:
+            return f"D[(q.{c0} + {ci}) * N + q.{r0} + {ri}]"
+        return f"D[(q.{r0} + {ri}) * N + q.{c0} + {ci}]"
+
+    def kupdates(G, trans, with_ad_bd):
+        # K(a,c) += X(b,d) ; K(b,c) += X(a,d)   (partial over the chunk's d)
         for ia in range(NA):
-            for ic in chunk:
+            for ic in cs:
                 em("{ double acc = 0.0;")
                 for ib in range(NB):
-                    for idd in range(ND):
-                        em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'd0', idd)};")
+                    for (jc, idd) in ks:
+                        if jc == ic:
+                            em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'd0', idd, trans)};")
                 em(f"  atomicAdd(&{G}[(q.a0 + {ia}) * N + q.c0 + {ic}], hk * acc); }}")
         for ib in range(NB):
-            for ic in chunk:
+            for ic in cs:
                 em("{ double acc = 0.0;")
                 for ia in range(NA):
-                    for idd in range(ND):
-                        em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'd0', idd)};")
+                    for (jc, idd) in ks:
+                        if jc == ic:
+                            em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'd0', idd, trans)};")
                 em(f"  atomicAdd(&{G}[(q.b0 + {ib}) * N + q.c0 + {ic}], hk * acc); }}")
-        return X
-
-    X = kblock("GK", False)
-    # persistent: K(a,d) += X(b,c) ; K(b,d) += X(a,c)
-    for ic in chunk:
-        for ib in range(NB):
-            em(f"{{ const double xbc = {X('b0', ib, 'c0', ic)};")
-            for ia in range(NA):
-                for idd in range(ND):
-                    em(f"  kad_{ia}_{idd}[s] += o_{ia}_{ib}_{ic}_{idd} * xbc;")
-            em("}")
+        if not with_ad_bd:
+            return
+        # K(a,d) += X(b,c) ; K(b,d) += X(a,c)   (partial over the chunk's c)
         for ia in range(NA):
-            em(f"{{ const double xac = {X('a0', ia, 'c0', ic)};")
+            for idd in ds:
+                em("{ double acc = 0.0;")
+                for ib in range(NB):
+                    for (ic, jd) in ks:
+                        if jd == idd:
+                            em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'c0', ic, trans)};")
+                em(f"  atomicAdd(&{G}[(q.a0 + {ia}) * N + q.d0 + {idd}], hk * acc); }}")
+        for ib in range(NB):
+            for idd in ds:
+                em("{ double acc = 0.0;")
+                for ia in range(NA):
+                    for (ic, jd) in ks:
+                        if jd == idd:
+                            em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'c0', ic, trans)};")
+                em(f"  atomicAdd(&{G}[(q.b0 + {ib}) * N + q.d0 + {idd}], hk * acc); }}")
+
+    kupdates("jk.GK[s]", False, not persist)
+    if persist:
+        for ic in cs:
             for ib in range(NB):
-                for idd in range(ND):
-                    em(f"  kbd_{ib}_{idd}[s] += o_{ia}_{ib}_{ic}_{idd} * xac;")
-            em("}")
+                em(f"{{ const double xbc = {X('b0', ib, 'c0', ic, False)};")
+                for ia in range(NA):
+                    for (jc, idd) in ks:
+                        if jc == ic:
+                            em(f"  kad_{ia}_{idd}[s] += o_{ia}_{ib}_{ic}_{idd} * xbc;")
+                em("}")
+            for ia in range(NA):
+                em(f"{{ const double xac = {X('a0', ia, 'c0', ic, False)};")
+                for ib in range(NB):
+                    for (jc, idd) in ks:
+                        if jc == ic:
+                            em(f"  kbd_{ib}_{idd}[s] += o_{ia}_{ib}_{ic}_{idd} * xac;")
+                em("}")
     em("if (ns) {")
     em.ind += 1
-    em("double* __restrict__ GK2 = jk.GK2[s];")
-    XT = kblock("GK2", True)
-    for ia in range(NA):
-        for idd in range(ND):
-            em("{ double acc = 0.0;")
-            for ib in range(NB):
-                for ic in chunk:
-                    em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {XT('b0', ib, 'c0', ic)};")
-            em(f"  atomicAdd(&GK2[(q.a0 + {ia}) * N + q.d0 + {idd}], hk * acc); }}")
-    for ib in range(NB):
-        for idd in range(ND):
-            em("{ double acc = 0.0;")
-            for ia in range(NA):
-                for ic in chunk:
-                    em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {XT('a0', ia, 'c0', ic)};")
-            em(f"  atomicAdd(&GK2[(q.b0 + {ib}) * N + q.d0 + {idd}], hk * acc); }}")
+    kupdates("jk.GK2[s]", True, True)
     em.ind -= 1
     em("}")
     em.ind -= 1
@@ -528,8 +549,7 @@ def main(out_dir: str, maxl: int = 2):
             src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, \"{name}\", "
                        f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}}};")
             src.append("}")
-            with open(os.path.join(out_dir, f"eri_{name}.hip"), "w") as f:
-                f.write("\n".join(src) + "\n")
+            _write_if_changed(os.path.join(out_dir, f"eri_{name}.hip"), "\n".join(src) + "\n")
             table.append(name)
     reg = ["// GENERATED by gen_eri.py -- do not edit."]
     for name in table:
@@ -539,8 +559,16 @@ def main(out_dir: str, maxl: int = 2):
     for k, name in enumerate(table):
         reg.append(f"  cq_register_{name}(t[{k}]);")
     reg.append("}")
-    with open(os.path.join(out_dir, "eri_registry.inc"), "w") as f:
-        f.write("\n".join(reg) + "\n")
+    _write_if_changed(os.path.join(out_dir, "eri_registry.inc"), "\n".join(reg) + "\n")
+
+
+def _write_if_changed(path: str, text: str):
+    """Leave an unchanged file untouched so incremental builds skip it."""
+    import os
+    if os.path.exists(path) and open(path).read() == text:
+        return
+    with open(path, "w") as f:
+        f.write(text)
 
 
 if __name__ == "__main__":

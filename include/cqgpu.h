@@ -88,6 +88,16 @@ int cq_get_quartet_counts(cq_ctx_t* ctx, long long* counts, int max_counts, int*
 int cq_num_class_pairs(void);
 int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld);
 
+/* Profiling (no reference counterpart beyond its wall-clock G[P] timer,
+ * include/singleslater/singleslater_fock.h:116-119): HIP events around every class-pair
+ * kernel; accumulated milliseconds and launch counts per class pair. */
+int cq_set_profiling(cq_ctx_t* ctx, int on);
+int cq_get_kernel_times(cq_ctx_t* ctx, double* ms, long long* launches, int max_n);
+
+/* Work sums of the last contraction per class pair, 4 doubles each: sum K_ab*K_cd
+ * (primitive quartets), sum K_ab, Cartesian->spherical transform FLOPs, output integrals. */
+int cq_get_work_stats(cq_ctx_t* ctx, double* out, int max_n);
+
 /* Contracted quartets (s1 s2|s3 s4) in the shells' own basis, row-major per quartet,
  * concatenated (the buffer libint2::Engine::compute returns, aointegrals_contract.h:138-143). */
 int cq_debug_quartets(cq_ctx_t* ctx, int nq, const int* shells4, double* out);

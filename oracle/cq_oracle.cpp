@@ -150,13 +150,14 @@ std::vector<double> car2sph(int l) {
   return M;
 }
 
+// Initialised once, thread-safely (C++11 function-local static), before any OpenMP region
+// can read it.
 const std::vector<double>& car2sph_cached(int l) {
-  static std::vector<double> cache[kMaxL + 1];
-  static bool init = false;
-  if (!init) {
-    for (int i = 0; i <= kMaxL; ++i) cache[i] = car2sph(i);
-    init = true;
-  }
+  static const std::vector<std::vector<double>> cache = [] {
+    std::vector<std::vector<double>> c(kMaxL + 1);
+    for (int i = 0; i <= kMaxL; ++i) c[i] = car2sph(i);
+    return c;
+  }();
   return cache[l];
 }
 

@@ -229,3 +229,22 @@ def test_golden_scf_energies(water):
         r = scf.rhf(water, _bs(rec["basis"], water), tau=1e-14)
         assert r["converged"]
         assert abs(r["E"] - rec["E"]) < d["tolerance"], (test, r["E"], rec["E"])
+
+
+def test_first_oracle_call_is_multithreaded():
+    """Regression: the Cartesian->spherical table is built once, thread-safely, even when the
+    first oracle call is the OpenMP-parallel contraction (fresh process)."""
+    import subprocess
+    import sys
+    code = (
+        "import numpy as np, oracle\n"
+        "from chronusq_public_amd import basis as B, molecule as M\n"
+        "mol = M.water_test0001(); bs = B.build_basis('6-31g*', mol.Z, mol.coords_bohr)\n"
+        "ob = oracle.OracleBasis(bs); N = bs.nbasis\n"
+        "X = np.eye(N); J = np.zeros((N, N), order='F'); K = np.zeros((N, N), order='F')\n"
+        "ob.contract_direct(np.ones((bs.nshell, bs.nshell)), 1e-12, [X, X], [J, K], [0, 1], [1, -0.5], nthreads=32)\n"
+        "print('ok')\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for _ in range(3):
+        r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr
