@@ -136,6 +136,23 @@ class AOIntegrals:
         rc = L.cq_contract_direct(self._h, n, xp, ap, _ip(types), _dp(scales), self.nBasis, self.nBasis)
         check(rc, self._h)
 
+    # ------------------------------------------------------------------ one-electron
+    def computeAOOneE(self, mol, device=None):
+        """S, T, V (torch float64 tensors on the GPU) -- AOIntegrals::computeAOOneE
+        (src/aointegrals/aointegrals_builders.cpp:337-467).  Symmetric, so the column-major
+        results are used as they are."""
+        import torch
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        N = self.nBasis
+        out = [torch.empty((N, N), dtype=torch.float64, device=dev) for _ in range(3)]
+        Z = np.ascontiguousarray(mol.Z, dtype=np.float64)
+        xyz = np.ascontiguousarray(mol.coords_bohr, dtype=np.float64)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        check(_lib.load().cq_one_electron(self._h, mol.natom, _dp(Z), _dp(xyz), ctypes.c_void_p(out[0].data_ptr()),
+                                          ctypes.c_void_p(out[1].data_ptr()), ctypes.c_void_p(out[2].data_ptr()),
+                                          N, ctypes.c_void_p(stream)), self._h)
+        return tuple(out)
+
     # ------------------------------------------------------------------ diagnostics
     def quartet_counts(self) -> dict:
         L = _lib.load()

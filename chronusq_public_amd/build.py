@@ -41,7 +41,7 @@ def build(incremental: bool = False, resource_usage: bool = False, jobs: int = 0
     os.makedirs(BUILD, exist_ok=True)
     gen_script = os.path.join(CSRC, "gen_eri.py")
     subprocess.run([sys.executable, gen_script, GEN], check=True)
-    srcs = sorted(glob.glob(os.path.join(GEN, "*.hip"))) + [os.path.join(CSRC, "gdirect.hip")]
+    srcs = sorted(glob.glob(os.path.join(GEN, "*.hip"))) + [os.path.join(CSRC, "gdirect.hip"), os.path.join(CSRC, "onee.hip")]
     deps = [os.path.join(CSRC, f) for f in ("gdirect_device.h", "gdirect_kernels.h")] + [gen_script]
     dep_mtime = max(os.path.getmtime(d) for d in deps)
     header_mtime = os.path.getmtime(os.path.join(HERE, "..", "include", "cqgpu.h"))
@@ -51,7 +51,8 @@ def build(incremental: bool = False, resource_usage: bool = False, jobs: int = 0
     for s in srcs:
         o = os.path.join(BUILD, os.path.basename(s) + ".o")
         objs.append(o)
-        newest = max(os.path.getmtime(s), dep_mtime, header_mtime if s.endswith("gdirect.hip") else 0.0)
+        newest = max(os.path.getmtime(s), dep_mtime, header_mtime if s.endswith("gdirect.hip") else 0.0,
+                     os.path.getmtime(os.path.join(CSRC, "onee.h")) if not s.startswith(GEN) else 0.0)
         if incremental and os.path.exists(o) and os.path.getmtime(o) > newest:
             continue
         todo.append((s, o))

@@ -39,6 +39,7 @@
 #include "../../include/cqgpu.h"
 
 #include "gdirect_kernels.h"
+#include "onee.h"
 #include "_gen/eri_registry.inc"
 
 static ClassKernels kClassKernels[CQ_NUM_CLASS_PAIRS];
@@ -206,6 +207,29 @@ __global__ void finalize_kernel(double* __restrict__ AX, int n, int ld, double f
     }
   }
   AX[(long)nu * ld + mu] += f * s;
+}
+
+// out[mu + nu*ld] = sum_{a,c} T(mu,a) T(nu,c) Mc[a][c]   (Cartesian row-major -> shells' basis)
+__global__ void sph_transform_kernel(const double* __restrict__ Mc, double* __restrict__ out, int n, int ld,
+                                     const ShellMeta* __restrict__ sh, const int* __restrict__ sph2shell, int ncart,
+                                     const double* __restrict__ tco, const int* __restrict__ tco_off) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)n * n) return;
+  const int mu = k % n, nu = k / n;
+  const ShellMeta A = sh[sph2shell[mu]], B = sh[sph2shell[nu]];
+  const int im = mu - A.sph_off, in = nu - B.sph_off;
+  const int nca = (A.l + 1) * (A.l + 2) / 2, ncb = (B.l + 1) * (B.l + 2) / 2;
+  double s = 0.0;
+  for (int a = 0; a < nca; ++a) {
+    const double ta = tcoef(A, im, a, tco, tco_off);
+    if (ta == 0.0) continue;
+    for (int c = 0; c < ncb; ++c) {
+      const double tc = tcoef(B, in, c, tco, tco_off);
+      if (tc == 0.0) continue;
+      s += ta * tc * Mc[(long)(A.cart_off + a) * ncart + B.cart_off + c];
+    }
+  }
+  out[(long)nu * ld + mu] = s;
 }
 
 // Schwarz: Q(P) = sqrt(max_{ij} |(ij|ij)|) in the shells' own (pure or Cartesian) basis,
@@ -443,6 +467,10 @@ struct cq_ctx {
   std::vector<double> tco;
   std::vector<int> tco_off;
   std::vector<double> boys_host;
+  std::vector<int> prim_off_host;
+  std::vector<double> alpha_host, coef_host, origin_host;
+  DevBuf<int> d_prim_off;
+  DevBuf<double> d_alpha, d_coef, d_origin, d_onee;
   // device
   DevBuf<ShellMeta> d_shells;
   DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start, d_flags;
@@ -459,6 +487,7 @@ struct cq_ctx {
     d_item_start.release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
     d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
     d_cps.release(); d_tasks.release(); d_counts.release(); d_nglob.release();
+    d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
@@ -569,6 +598,14 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   flatten();
   const int npair = (int)c->pairs.size();
   // device uploads
+  c->prim_off_host = poff;
+  c->alpha_host.assign(alpha, alpha + poff[nshell]);
+  c->coef_host.assign(coef, coef + poff[nshell]);
+  c->origin_host.assign(origin, origin + 3 * nshell);
+  c->d_prim_off.upload(c->prim_off_host, c->stream);
+  c->d_alpha.upload(c->alpha_host, c->stream);
+  c->d_coef.upload(c->coef_host, c->stream);
+  c->d_origin.upload(c->origin_host, c->stream);
   c->d_shells.upload(c->shells, c->stream);
   c->d_cart2shell.upload(c->cart2shell, c->stream);
   c->d_sph2shell.upload(c->sph2shell, c->stream);
@@ -953,6 +990,43 @@ int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld_) {
 }
 
 int cq_num_class_pairs(void) { return kNumClassPairs; }
+
+// S, T, V in the shells' basis (column-major n x n, leading dimension ld, device memory),
+// replacing AOIntegrals::computeAOOneE (src/aointegrals/aointegrals_builders.cpp:337-467).
+// Z[natom] nuclear charges and xyz[3*natom] positions (bohr) are host arrays.
+int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, double* S, double* T, double* V,
+                    int ld, void* stream) {
+  return guarded(c, [&] {
+    if (natom < 0 || (natom > 0 && (!Z || !xyz)) || !S || !T || !V || ld < c->nsph)
+      throw std::invalid_argument("invalid one-electron arguments");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const long nc2 = (long)c->ncart * c->ncart;
+    c->d_onee.ensure((size_t)3 * nc2 + 4 * (size_t)std::max(1, natom));
+    double* Sc = c->d_onee.p;
+    double* Tc = Sc + nc2;
+    double* Vc = Tc + nc2;
+    double* dZ = Vc + nc2;
+    double* dxyz = dZ + std::max(1, natom);
+    std::vector<double> zx((size_t)4 * std::max(1, natom), 0.0);
+    for (int a = 0; a < natom; ++a) {
+      zx[a] = Z[a];
+      for (int k = 0; k < 3; ++k) zx[std::max(1, natom) + 3 * a + k] = xyz[3 * a + k];
+    }
+    HIPCHK(hipMemcpyAsync(dZ, zx.data(), zx.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    OneEShellData sd{c->d_prim_off.p, c->d_alpha.p, c->d_coef.p, c->d_origin.p};
+    for (int X = 0; X < c->npc; ++X)
+      launch_onee(X, c->d_pairs.p, c->class_start[X], c->class_start[X + 1], sd, dZ, dxyz, natom, c->d_boys.p,
+                  c->ncart, Sc, Tc, Vc, s);
+    HIPCHK(hipGetLastError());
+    double* outs[3] = {S, T, V};
+    for (int k = 0; k < 3; ++k)
+      hipLaunchKernelGGL(sph_transform_kernel, dim3(grid_for((long)c->nsph * c->nsph, WG)), dim3(WG), 0, s,
+                         c->d_onee.p + k * nc2, outs[k], c->nsph, ld, c->d_shells.p, c->d_sph2shell.p, c->ncart,
+                         c->d_tco.p, c->d_tco_off.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));   // the host staging vector is released on return
+  });
+}
 
 // Per-class-pair kernel timing with HIP events recorded on the launch stream.
 int cq_set_profiling(cq_ctx_t* c, int on) {

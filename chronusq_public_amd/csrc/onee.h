@@ -1,0 +1,19 @@
+// One-electron integral launcher shared between onee.hip and the host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "gdirect_device.h"
+
+// Raw shell data on the device: primitive ranges prim_off[s]..prim_off[s+1] into alpha/coef,
+// centres origin[3*s] (bohr).
+struct OneEShellData {
+  const int* prim_off;
+  const double* alpha;
+  const double* coef;
+  const double* origin;
+};
+
+// S, T, V Cartesian blocks of the shell pairs [p_begin, p_end) of one pair class
+// (0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd), written into row-major ncart x ncart matrices.
+void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
+                 const double* Z, const double* xyz, int natom, const double* boys, int ncart, double* S, double* T,
+                 double* V, hipStream_t stream);

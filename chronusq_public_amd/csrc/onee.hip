@@ -1,0 +1,206 @@
+// One-electron integrals on the GPU: overlap S, kinetic T, nuclear attraction V
+// (replaces AOIntegrals::computeAOOneE / OneEDriver, src/aointegrals/aointegrals_builders.cpp:
+// 27-132,337-467).  McMurchie-Davidson: Hermite expansion coefficients by their closed form
+//   E^{ij}_t = sum_{k<=i,l<=j} C(i,k) C(j,l) PA^{i-k} PB^{j-l} n!/(t! m!) (1/4p)^m (1/2p)^t,
+//   n = k + l, m = (n - t)/2,
+// S_1D(i,j) = E^{ij}_0 sqrt(pi/p),
+// T_1D(i,j) = -1/2 [ j(j-1) S(i,j-2) - 2b(2j+1) S(i,j) + 4b^2 S(i,j+2) ],
+// V = -(2 pi / p) sum_C Z_C sum_tuv E_t E_u E_v R_tuv(p, P - C).
+// One thread per shell pair (l_a >= l_b), every loop unrolled at compile time; S and T in one
+// pass, V in a second pass that first sums R_tuv over the nuclei.  The Cartesian blocks are
+// written to row-major ncart x ncart matrices (and their transposes).
+#include "gdirect_kernels.h"
+#include "onee.h"
+
+namespace {
+
+__host__ __device__ constexpr int comp_row(int k) {
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= k) ++i;
+  return i;
+}
+__host__ __device__ constexpr int comp_x(int l, int k) { return l - comp_row(k); }
+__host__ __device__ constexpr int comp_y(int l, int k) { return comp_row(k) - (k - comp_row(k) * (comp_row(k) + 1) / 2); }
+__host__ __device__ constexpr int comp_z(int l, int k) { return l - comp_x(l, k) - comp_y(l, k); }
+__host__ __device__ constexpr double cfact(int n) { return n <= 1 ? 1.0 : n * cfact(n - 1); }
+__host__ __device__ constexpr double cbinom(int n, int k) { return cfact(n) / (cfact(k) * cfact(n - k)); }
+
+__device__ __forceinline__ double ipow(double x, int n) {
+  double r = 1.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k < n) r *= x;
+  return r;
+}
+
+// E^{ij}_t for one Cartesian direction (without the exp(-mu AB^2) factor).
+__device__ __forceinline__ double hermE(int i, int j, int t, double PA, double PB, double o2p) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k <= 6; ++k) {
+    if (k > i) break;
+#pragma unroll
+    for (int l = 0; l <= 6; ++l) {
+      if (l > j) break;
+      const int n = k + l;
+      if (n < t || ((n - t) & 1)) continue;
+      const int m = (n - t) / 2;
+      const double c = cbinom(i, k) * cbinom(j, l) * cfact(n) / (cfact(t) * cfact(m));
+      s += c * ipow(PA, i - k) * ipow(PB, j - l) * ipow(0.5 * o2p, m) * ipow(o2p, t);
+    }
+  }
+  return s;
+}
+
+template <int LA, int LB, bool DO_V>
+__global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ pairs, int p_begin, int p_end,
+                                                  const OneEShellData sd, const double* __restrict__ Zc,
+                                                  const double* __restrict__ xyz, int natom,
+                                                  const double* __restrict__ boys, int ncart,
+                                                  double* __restrict__ S, double* __restrict__ T,
+                                                  double* __restrict__ V) {
+  constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2, L = LA + LB;
+  __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];
+  if (DO_V) stage_boys(boys, L, boys_lds);
+  const int P = p_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= p_end) return;
+  const PairInfo pi = pairs[P];
+  const int a = pi.a, b = pi.b;
+  const double A[3] = {sd.origin[3 * a], sd.origin[3 * a + 1], sd.origin[3 * a + 2]};
+  const double B[3] = {sd.origin[3 * b], sd.origin[3 * b + 1], sd.origin[3 * b + 2]};
+  const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+  double s[NA][NB], t[NA][NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) { s[i][j] = 0.0; t[i][j] = 0.0; }
+  for (int ip = sd.prim_off[a]; ip < sd.prim_off[a + 1]; ++ip)
+    for (int jp = sd.prim_off[b]; jp < sd.prim_off[b + 1]; ++jp) {
+      const double al = sd.alpha[ip], be = sd.alpha[jp];
+      const double p = al + be, o2p = 0.5 / p;
+      const double K = exp(-al * be / p * AB2) * sd.coef[ip] * sd.coef[jp];
+      const double PA0 = be / p * (B[0] - A[0]), PA1 = be / p * (B[1] - A[1]), PA2 = be / p * (B[2] - A[2]);
+      const double PB0 = al / p * (A[0] - B[0]), PB1 = al / p * (A[1] - B[1]), PB2 = al / p * (A[2] - B[2]);
+      if (!DO_V) {
+        const double sq = sqrt(3.14159265358979323846 / p);
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+            const int ax = comp_x(LA, ia), ay = comp_y(LA, ia), az = comp_z(LA, ia);
+            const int bx = comp_x(LB, ib), by = comp_y(LB, ib), bz = comp_z(LB, ib);
+            const double sx = hermE(ax, bx, 0, PA0, PB0, o2p) * sq;
+            const double sy = hermE(ay, by, 0, PA1, PB1, o2p) * sq;
+            const double sz = hermE(az, bz, 0, PA2, PB2, o2p) * sq;
+            auto kin = [&](int i, int j, double PAd, double PBd, double sij) {
+              double d2 = 4.0 * be * be * hermE(i, j + 2, 0, PAd, PBd, o2p) * sq - 2.0 * be * (2 * j + 1) * sij;
+              if (j >= 2) d2 += (double)(j * (j - 1)) * hermE(i, j - 2, 0, PAd, PBd, o2p) * sq;
+              return -0.5 * d2;
+            };
+            s[ia][ib] += K * sx * sy * sz;
+            t[ia][ib] += K * (kin(ax, bx, PA0, PB0, sx) * sy * sz + sx * kin(ay, by, PA1, PB1, sy) * sz +
+                              sx * sy * kin(az, bz, PA2, PB2, sz));
+          }
+      } else {
+        const double Px = A[0] + PA0, Py = A[1] + PA1, Pz = A[2] + PA2;
+        // Rsum_tuv = sum_C Z_C R_tuv(p, P - C), t + u + w <= L
+        double Rs[L + 1][L + 1][L + 1];
+#pragma unroll
+        for (int tt = 0; tt <= L; ++tt)
+#pragma unroll
+          for (int u = 0; u <= L; ++u)
+#pragma unroll
+            for (int w = 0; w <= L; ++w) Rs[tt][u][w] = 0.0;
+        for (int c = 0; c < natom; ++c) {
+          const double X = Px - xyz[3 * c], Y = Py - xyz[3 * c + 1], Z = Pz - xyz[3 * c + 2];
+          double F[L + 1];
+          boys_eval<L>(p * (X * X + Y * Y + Z * Z), boys_lds, F);
+          double R[L + 1][L + 1][L + 1][L + 1];
+          double fac = Zc[c];
+#pragma unroll
+          for (int n = 0; n <= L; ++n) { R[n][0][0][0] = fac * F[n]; fac *= -2.0 * p; }
+#pragma unroll
+          for (int n = L - 1; n >= 0; --n)
+#pragma unroll
+            for (int tt = 0; tt <= L; ++tt)
+#pragma unroll
+              for (int u = 0; u <= L; ++u)
+#pragma unroll
+                for (int w = 0; w <= L; ++w) {
+                  if (tt + u + w == 0 || n + tt + u + w > L) continue;
+                  double r;
+                  if (tt > 0) r = X * R[n + 1][tt - 1][u][w] + (tt > 1 ? (tt - 1) * R[n + 1][tt - 2][u][w] : 0.0);
+                  else if (u > 0) r = Y * R[n + 1][tt][u - 1][w] + (u > 1 ? (u - 1) * R[n + 1][tt][u - 2][w] : 0.0);
+                  else r = Z * R[n + 1][tt][u][w - 1] + (w > 1 ? (w - 1) * R[n + 1][tt][u][w - 2] : 0.0);
+                  R[n][tt][u][w] = r;
+                }
+#pragma unroll
+          for (int tt = 0; tt <= L; ++tt)
+#pragma unroll
+            for (int u = 0; u <= L; ++u)
+#pragma unroll
+              for (int w = 0; w <= L; ++w)
+                if (tt + u + w <= L) Rs[tt][u][w] += R[0][tt][u][w];
+        }
+        const double pref = -2.0 * 3.14159265358979323846 / p * K;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+            const int ax = comp_x(LA, ia), ay = comp_y(LA, ia), az = comp_z(LA, ia);
+            const int bx = comp_x(LB, ib), by = comp_y(LB, ib), bz = comp_z(LB, ib);
+            double acc = 0.0;
+#pragma unroll
+            for (int tt = 0; tt <= ax + bx; ++tt) {
+              const double ex = hermE(ax, bx, tt, PA0, PB0, o2p);
+#pragma unroll
+              for (int u = 0; u <= ay + by; ++u) {
+                const double ey = hermE(ay, by, u, PA1, PB1, o2p);
+#pragma unroll
+                for (int w = 0; w <= az + bz; ++w) acc += ex * ey * hermE(az, bz, w, PA2, PB2, o2p) * Rs[tt][u][w];
+              }
+            }
+            s[ia][ib] += pref * acc;   // V accumulates in s in this pass
+          }
+      }
+    }
+  double* M0 = DO_V ? V : S;
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      const long r = pi.a0 + ia, cidx = pi.b0 + ib;
+      M0[r * ncart + cidx] = s[ia][ib];
+      M0[cidx * ncart + r] = s[ia][ib];
+      if (!DO_V) {
+        T[r * ncart + cidx] = t[ia][ib];
+        T[cidx * ncart + r] = t[ia][ib];
+      }
+    }
+}
+
+}  // namespace
+
+void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
+                 const double* Z, const double* xyz, int natom, const double* boys, int ncart, double* S, double* T,
+                 double* V, hipStream_t stream) {
+  const int n = p_end - p_begin;
+  if (n <= 0) return;
+  const dim3 grid((n + WG - 1) / WG), block(WG);
+#define CQ_ONEE(LA, LB)                                                                                 \
+  do {                                                                                                   \
+    hipLaunchKernelGGL((onee_kernel<LA, LB, false>), grid, block, 0, stream, pairs, p_begin, p_end, sd, Z, xyz, \
+                       natom, boys, ncart, S, T, V);                                                     \
+    hipLaunchKernelGGL((onee_kernel<LA, LB, true>), grid, block, 0, stream, pairs, p_begin, p_end, sd, Z, xyz,  \
+                       natom, boys, ncart, S, T, V);                                                     \
+  } while (0)
+  switch (pair_class) {
+    case 0: CQ_ONEE(0, 0); break;
+    case 1: CQ_ONEE(1, 0); break;
+    case 2: CQ_ONEE(1, 1); break;
+    case 3: CQ_ONEE(2, 0); break;
+    case 4: CQ_ONEE(2, 1); break;
+    case 5: CQ_ONEE(2, 2); break;
+    default: break;
+  }
+}

@@ -1,0 +1,246 @@
+"""Device-resident Hartree-Fock driver around the GPU G[D] (mirror of ChronusQ's
+``SingleSlater<double>`` for real RHF / UHF).
+
+Everything stays in FP64 on the GPU: one-electron integrals and G[D] come from libcqgpu.so,
+the dense linear algebra (orthonormalisation, CDIIS, eigensolve) runs through torch on the
+same device.  Steps and conventions follow the reference:
+
+* ``formPT`` (include/singleslater/singleslater_fock.h:30-93): RHF X = {P, P},
+  types = {COULOMB, EXCHANGE}, scales = {1, -1/2}, so PT = 2 J[P] - K[P]; UHF adds
+  {Mz, EXCHANGE, -1/2}, PT_z = -K[Mz].
+* ``formFock`` (singleslater_fock.h:99-274): F_S = 2 H + PT (the orthonormal Fock that is
+  diagonalised is 1/2 X F_S X, singleslater_scfutils.h:567-569); here F = H + PT/2 directly.
+* ``computeEnergy`` (singleslater_properties.h:25-112): E = Tr(P H) + 1/4 Tr(P PT) + E_nuc
+  (UHF adds 1/4 Tr(Mz PT_z)).
+* ``formDensity`` (singleslater_misc.h:27-56): P = 2 C_occ C_occ^T (RHF), P_a + P_b / P_a - P_b
+  (UHF).
+* CDIIS (singleslater_diis.h:34-156, include/extrapolate.h:60-142): depth 10, orthonormal
+  commutator FP - PF; a singular B system falls back to the plain step.
+* damping (singleslater_scfutils.h:1065-1111): F_o <- (1-d) F_o + d F_o(prev) while
+  |dE| > 1e-6 (off by default here: it only lengthens the path to the same energy).
+* convergence (singleslater_scfutils.h:262-355): |dE| < eneTol and ||dP||_F < denTol
+  (1e-10, 1e-8, include/singleslater.h:448-449); non-convergence raises.
+* SAD guess (singleslater_guess.h:99-222): one UHF per unique element at its default
+  multiplicity, atomic scalar densities on the diagonal blocks, spin-averaged, one Fock build
+  and diagonalisation.  The atomic UHFs use CDIIS (the reference switches DIIS off for them).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import basis as B
+from .aointegrals import AOIntegrals, COULOMB, EXCHANGE
+from .molecule import Molecule
+
+# ground-state multiplicities used for the atomic guesses
+DEFAULT_MULT = {1: 2, 2: 1, 3: 2, 4: 1, 5: 2, 6: 3, 7: 4, 8: 3, 9: 2, 10: 1}
+
+
+class SCFNotConverged(RuntimeError):
+    """SCF failed to converge within maxSCFIter (singleslater_scf.h:253-255)."""
+
+
+@dataclass
+class SCFResult:
+    energy: float
+    iterations: int
+    converged: bool
+    energies: List[float] = field(default_factory=list)
+    times: List[float] = field(default_factory=list)       # seconds since SCF start, per iteration
+    gd_times: List[float] = field(default_factory=list)    # seconds per G[D]
+    P: object = None
+    Mz: object = None
+    setup_s: float = 0.0
+
+
+class SingleSlater:
+    def __init__(self, mol: Molecule, basis, ref: str = "RHF", tau: float = 1e-12, guess: str = "SAD",
+                 ndiis: int = 10, damp: float = 0.0, ene_tol: float = 1e-10, den_tol: float = 1e-8,
+                 max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None):
+        import torch
+        self.torch = torch
+        self.mol = mol
+        self.basis = basis if isinstance(basis, B.BasisSet) else B.build_basis(basis, mol.Z, mol.coords_bohr)
+        self.ref = ref.upper()
+        self.guess = guess.upper()
+        self.ndiis, self.damp = ndiis, damp
+        self.ene_tol, self.den_tol, self.max_iter = ene_tol, den_tol, max_iter
+        self.dev = device or torch.device("cuda", torch.cuda.current_device())
+        t0 = time.perf_counter()
+        self.ao = aoints or AOIntegrals(self.basis)
+        self.ao.thresholdSchwartz = tau
+        S, T, V = self.ao.computeAOOneE(mol, self.dev)
+        self.S, self.H = S, T + V
+        w, U = torch.linalg.eigh(S)
+        self.X = (U * w.rsqrt()) @ U.T          # S^{-1/2} (Loewdin, aointegrals_builders.cpp:736-769)
+        self.Xi = (U * w.sqrt()) @ U.T          # S^{1/2}
+        self.enuc = mol.nuclear_repulsion()
+        nel = mol.nelectron
+        if self.ref == "RHF":
+            if nel % 2:
+                raise ValueError("RHF needs an even electron count")
+            self.nocc = (nel // 2, nel // 2)
+        else:
+            na = (nel + mol.mult - 1) // 2
+            self.nocc = (na, nel - na)
+        self.N = self.basis.nbasis
+        self.setup_s = time.perf_counter() - t0
+
+    # ------------------------------------------------------------------ pieces
+    def formPT(self, P, Mz=None):
+        torch = self.torch
+        N = self.N
+        J = torch.zeros((N, N), dtype=torch.float64, device=self.dev)
+        K = torch.zeros_like(J)
+        if Mz is None:
+            self.ao.newTwoEContractDirect([P, P], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+            return J + K, None
+        Kz = torch.zeros_like(J)
+        self.ao.newTwoEContractDirect([P, P, Mz], [J, K, Kz], [COULOMB, EXCHANGE, EXCHANGE], [1.0, -0.5, -0.5])
+        return J + K, Kz
+
+    def _diag_density(self, F, nocc):
+        torch = self.torch
+        Fo = self.X @ F @ self.X
+        e, C = torch.linalg.eigh(Fo)
+        Cao = self.X @ C[:, :nocc]
+        return Cao @ Cao.T, e
+
+    def _energy(self, P, PT, Mz=None, PTz=None):
+        E = float((P * self.H).sum() + 0.25 * (P * PT).sum()) + self.enuc
+        if Mz is not None:
+            E += 0.25 * float((Mz * PTz).sum())
+        return E
+
+    def core_guess(self):
+        na, nb = self.nocc
+        Pa, _ = self._diag_density(self.H, na)
+        if self.ref == "RHF":
+            return 2.0 * Pa, None
+        Pb, _ = self._diag_density(self.H, nb)
+        return Pa + Pb, Pa - Pb
+
+    def sad_guess(self):
+        """Superposition of atomic (UHF) densities, spin-averaged, then one Fock build and
+        diagonalisation (singleslater_guess.h:24-65,99-222)."""
+        torch = self.torch
+        if self.mol.natom <= 1:
+            return self.core_guess()
+        atom_of_bf = np.concatenate([[s.atom] * s.size for s in self.basis.shells])
+        Ps = torch.zeros((self.N, self.N), dtype=torch.float64, device=self.dev)
+        cache: Dict[int, object] = {}
+        for z in dict.fromkeys(int(v) for v in self.mol.Z):
+            atom = Molecule(np.array([z]), np.zeros((1, 3)), 0, DEFAULT_MULT.get(z, 1))
+            hf = SingleSlater(atom, self.basis.name, ref="UHF", tau=1e-14, guess="CORE", ndiis=self.ndiis,
+                              max_iter=512, device=self.dev)
+            r = hf.SCF3()
+            cache[z] = r.P
+        for iat, z in enumerate(self.mol.Z):
+            idx = np.nonzero(atom_of_bf == iat)[0]
+            lo, hi = int(idx[0]), int(idx[-1]) + 1
+            Ps[lo:hi, lo:hi] = cache[int(z)]
+        PT, _ = self.formPT(Ps)
+        F = self.H + 0.5 * PT
+        na, nb = self.nocc
+        Pa, _ = self._diag_density(F, na)
+        if self.ref == "RHF":
+            return 2.0 * Pa, None
+        Pb, _ = self._diag_density(F, nb)
+        return Pa + Pb, Pa - Pb
+
+    # ------------------------------------------------------------------ SCF3
+    def SCF3(self, P0=None, Mz0=None, reference_energy: Optional[float] = None) -> SCFResult:
+        torch = self.torch
+        t_start = time.perf_counter()
+        if P0 is None:
+            P, Mz = self.sad_guess() if self.guess == "SAD" else self.core_guess()
+        else:
+            P, Mz = P0, Mz0
+        uhf = self.ref != "RHF"
+        na, nb = self.nocc
+        hist_F: List[tuple] = []
+        hist_e: List[torch.Tensor] = []
+        res = SCFResult(energy=0.0, iterations=0, converged=False, setup_s=self.setup_s)
+        E_old = 0.0
+        Fo_prev = None
+        damp = self.damp
+        for it in range(self.max_iter):
+            torch.cuda.synchronize()
+            tg = time.perf_counter()
+            PT, PTz = self.formPT(P, Mz if uhf else None)
+            torch.cuda.synchronize()
+            res.gd_times.append(time.perf_counter() - tg)
+            E = self._energy(P, PT, Mz, PTz)
+            if uhf:
+                Fa = self.H + 0.5 * (PT + PTz)
+                Fb = self.H + 0.5 * (PT - PTz)
+                Pa, Pb = 0.5 * (P + Mz), 0.5 * (P - Mz)
+                errs = [self.X @ (Fa @ Pa @ self.S - self.S @ Pa @ Fa) @ self.X,
+                        self.X @ (Fb @ Pb @ self.S - self.S @ Pb @ Fb) @ self.X]
+                Fs = (Fa, Fb)
+            else:
+                F = self.H + 0.5 * PT
+                errs = [self.X @ (F @ P @ self.S - self.S @ P @ F) @ self.X]
+                Fs = (F,)
+            if self.ndiis > 0:
+                hist_F.append(Fs)
+                hist_e.append(torch.cat([e.reshape(-1) for e in errs]))
+                if len(hist_F) > self.ndiis:
+                    hist_F.pop(0)
+                    hist_e.pop(0)
+                n = len(hist_F)
+                if n > 1:
+                    E_mat = torch.stack(hist_e)
+                    Bm = -torch.ones((n + 1, n + 1), dtype=torch.float64, device=self.dev)
+                    Bm[:n, :n] = E_mat @ E_mat.T
+                    Bm[n, n] = 0.0
+                    rhs = torch.zeros(n + 1, dtype=torch.float64, device=self.dev)
+                    rhs[n] = -1.0
+                    c = torch.linalg.solve(Bm, rhs)[:n]
+                    if bool(torch.isfinite(c).all()):
+                        Fs = tuple(sum(c[i] * hist_F[i][k] for i in range(n)) for k in range(len(Fs)))
+            if damp > 0.0 and Fo_prev is not None:
+                Fs = tuple((1 - damp) * f + damp * fp for f, fp in zip(Fs, Fo_prev))
+            Fo_prev = Fs
+            if uhf:
+                Pa_n, _ = self._diag_density(Fs[0], na)
+                Pb_n, _ = self._diag_density(Fs[1], nb)
+                Pn, Mzn = Pa_n + Pb_n, Pa_n - Pb_n
+            else:
+                Pa_n, _ = self._diag_density(Fs[0], na)
+                Pn, Mzn = 2.0 * Pa_n, None
+            dP = float(torch.linalg.norm(Pn - P))
+            if uhf:
+                dP = max(dP, float(torch.linalg.norm(Mzn - Mz)))
+            dE = E - E_old
+            E_old = E
+            if abs(dE) < 1e-6:
+                damp = 0.0
+            P, Mz = Pn, Mzn
+            res.energies.append(E)
+            res.times.append(time.perf_counter() - t_start)
+            res.iterations = it + 1
+            if it > 0 and abs(dE) < self.ene_tol and dP < self.den_tol:
+                res.converged = True
+                break
+        if not res.converged:
+            raise SCFNotConverged("SCF Failed to converge within MAXITER iterations")
+        res.energy = res.energies[-1]
+        res.P, res.Mz = P, Mz
+        return res
+
+
+def time_to_energy(res: SCFResult, E_ref: float, tol: float = 1e-6) -> float:
+    """Seconds from SCF start until |E - E_ref| <= tol holds for this and every later iteration."""
+    t = None
+    for E, ts in zip(res.energies, res.times):
+        if abs(E - E_ref) <= tol:
+            if t is None:
+                t = ts
+        else:
+            t = None
+    return float("nan") if t is None else t

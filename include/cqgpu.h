@@ -83,6 +83,12 @@ int cq_contract_direct(cq_ctx_t* ctx, int nmat, const double* const* X, double* 
 int cq_contract_direct_device(cq_ctx_t* ctx, int nmat, const double* const* X, double* const* AX,
                               const int* type, const double* scale, int n, int ld, void* stream);
 
+/* One-electron integrals S, T, V in the shells' basis, column-major n x n (leading dimension
+ * ld) in device memory; Z[natom] and xyz[3*natom] (bohr) on the host.  Replaces
+ * AOIntegrals::computeAOOneE (src/aointegrals/aointegrals_builders.cpp:337-467).  Synchronous. */
+int cq_one_electron(cq_ctx_t* ctx, int natom, const double* Z, const double* xyz, double* S, double* T,
+                    double* V, int ld, void* stream);
+
 /* Surviving quartets per class pair of the last contraction (synchronises). */
 int cq_get_quartet_counts(cq_ctx_t* ctx, long long* counts, int max_counts, int* n_out);
 int cq_num_class_pairs(void);

@@ -3,6 +3,7 @@
 * boys.json: F_m(T) = Gamma(m+1/2) P(m+1/2, T) / (2 T^(m+1/2)) from scipy's regularised
   incomplete gamma function (F_m(0) = 1/(2m+1)), m = 0..16, T on a grid covering the
   Taylor and asymptotic regimes.
+* energies_uhf.json: the UHF goldens test0007-0012 (Li doublet, O2 triplet).
 * energies.json: the reference's golden RHF energies, read as text from
   oldTests.tgz:tests/chronus-ref.val (test0001-0003; tolerance 5e-10 from tests/rununit.py:39),
   with the geometry of tests/test0001.inp.
@@ -46,6 +47,21 @@ def main():
         if v[0] in basis and v[-1].strip() == "SCF":
             energies[v[0]] = {"basis": basis[v[0]], "E": float(v[1]),
                               "dipole": [float(x) for x in v[2:5]]}
+    uhf = {"test0007": ("sto-3g", "Li"), "test0008": ("6-31g", "Li"), "test0009": ("cc-pvdz", "Li"),
+           "test0010": ("sto-3g", "O2"), "test0011": ("6-31g", "O2"), "test0012": ("cc-pvdz", "O2")}
+    uhf_rec = {}
+    for line in txt.splitlines():
+        v = line.split("/")
+        if v[0] in uhf and v[-1].strip() == "SCF":
+            uhf_rec[v[0]] = {"basis": uhf[v[0]][0], "system": uhf[v[0]][1], "E": float(v[1])}
+    out_uhf = {"source": "oldTests.tgz:tests/chronus-ref.val, decks oldTests.tgz:tests/test0007-0012_serial_direct.inp",
+               "tolerance": 5e-10,
+               "systems": {"Li": {"symbols": ["Li"], "angstrom": [[0.0, 0.0, 0.0]], "mult": 2},
+                           "O2": {"symbols": ["O", "O"], "angstrom": [[0.0, 0.0, 0.608586], [0.0, 0.0, -0.608586]],
+                                  "mult": 3}},
+               "tests": uhf_rec}
+    with open(os.path.join(HERE, "energies_uhf.json"), "w") as f:
+        json.dump(out_uhf, f, indent=1)
     out = {"source": "oldTests.tgz:tests/chronus-ref.val", "tolerance": 5e-10,
            "geometry_angstrom": {"O": [0.0, -0.07579184359, 0.0], "H1": [0.866811829, 0.6014357793, 0.0],
                                  "H2": [-0.866811829, 0.6014357793, 0.0]},
