@@ -127,7 +127,7 @@ def main():
     torch.cuda.synchronize()
     L = _lib.load()
     ncp = L.cq_num_class_pairs()
-    _lib.check(L.cq_set_profiling(ao._h, 1), ao._h)
+    # timed region: the production configuration (class kernels over 4 streams)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -139,10 +139,17 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = e0.elapsed_time(e1) / args.steps
+    # per-kernel pass: kernels serialised on one stream, HIP events around each launch
+    _lib.check(L.cq_set_launch_streams(ao._h, 1), ao._h)
+    _lib.check(L.cq_set_profiling(ao._h, 1), ao._h)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
     ms = (ctypes.c_double * ncp)()
     nl = (ctypes.c_longlong * ncp)()
     _lib.check(L.cq_get_kernel_times(ao._h, ms, nl, ncp), ao._h)
     _lib.check(L.cq_set_profiling(ao._h, 0), ao._h)
+    _lib.check(L.cq_set_launch_streams(ao._h, 4), ao._h)
     counts = ao.quartet_counts()
     nquart = sum(counts.values())
     ws = (ctypes.c_double * (4 * ncp))()
@@ -171,7 +178,9 @@ def main():
         "config": {"workload": "RHF G[D] = 2J[P] - K[P], (H2O)16 seed 0 / cc-pVDZ", "nbf": N,
                    "nshell": bs.nshell, "tau": args.tau, "surviving_quartets": nquart,
                    "model_gflop_per_step": total_flops / 1e9,
-                   "all_kernels_tflops": total_flops / (sum(r["kernel_ms"] for r in per_class) * 1e-3) / 1e12},
+                   "serial_kernels_ms": sum(r["kernel_ms"] for r in per_class),
+                   "all_kernels_tflops": total_flops / (sum(r["kernel_ms"] for r in per_class) * 1e-3) / 1e12,
+                   "step_tflops": total_flops / (ms_per_step * 1e-3) / 1e12},
         "roofline": {"bound": "mfma", "pipe": "FP64 VALU (MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s)", "kernel": "eri_jk" + dom["class"], "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": None},

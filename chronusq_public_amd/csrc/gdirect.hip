@@ -50,60 +50,84 @@ static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 // Screening: one wavefront per (class pair, bra pair); compacted task lists.
 // ---------------------------------------------------------------------------------------
 struct CPMeta {
-  int bra_start, nbra, ket_start, nket, same, pad;
+  int bra_start, nbra, ket_start, nket, same, bra_class;
   long long task_off;
 };
 
+// One wavefront per (class pair, ket pair); lanes sweep the bra pairs (sorted by Schwarz
+// bound, so the sweep stops once no bra can pass).  Survivors are staged in a per-wavefront
+// LDS buffer and flushed with one atomic per buffer, so task lists come out ket-major: the
+// lanes of an ERI wavefront then share one ket pair.
+#define SCREEN_BUF 1024
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __global__ void __launch_bounds__(WG) screen_kernel(
     const CPMeta* __restrict__ cps, int ncp, const int* __restrict__ item_start, int nitem,
-    const PairInfo* __restrict__ pairs, const double* __restrict__ pairQ,
-    const double* __restrict__ Nmax, const unsigned long long* __restrict__ nglob_bits, int nshell,
-    double tau, uint2* __restrict__ tasks, unsigned long long* __restrict__ counts) {
-  const int wave = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+    const int2* __restrict__ s12, const double* __restrict__ pairQ, const int* __restrict__ grp,
+    const int* __restrict__ cls_grp_off, const double* __restrict__ Nmax,
+    const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, uint2* __restrict__ tasks,
+    unsigned long long* __restrict__ counts) {
+  __shared__ uint2 buf[WG / 64][SCREEN_BUF];
+  const int wid = threadIdx.x >> 6;
+  const int wave = blockIdx.x * (WG / 64) + wid;
   const int lane = threadIdx.x & 63;
   if (wave >= nitem) return;
   int cp = 0;
   while (cp + 1 < ncp && item_start[cp + 1] <= wave) ++cp;
   const CPMeta m = cps[cp];
-  const int bra_local = wave - item_start[cp];
-  const int P = m.bra_start + bra_local;
-  const int ket_end = m.same ? bra_local + 1 : m.nket;
-  const double Qp = pairQ[P];
-  const PairInfo pb = pairs[P];
-  const long canonP = (long)pb.s1 * (pb.s1 + 1) / 2 + pb.s2;
+  const int ket_local = wave - item_start[cp];
+  const int K = m.ket_start + ket_local;
+  const int i_begin = m.same ? ket_local : 0;
+  const double Qk = pairQ[K];
+  const int2 sk = s12[K];
+  const long canonK = (long)sk.x * (sk.x + 1) / 2 + sk.y;
   const double nglob = __longlong_as_double((long long)*nglob_bits);
-  for (int j0 = 0; j0 < ket_end; j0 += 64) {
-    const double Qj0 = pairQ[m.ket_start + j0];
-    if (nglob * Qp * Qj0 * (1.0 + 1e-12) < tau) break;
-    const int j = j0 + lane;
+  uint2* wbuf = buf[wid];
+  int nb = 0;
+  auto flush = [&]() {
+    wave_sync_lds();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&counts[cp], (unsigned long long)nb);
+    base = __shfl(base, 0);
+    for (int k = lane; k < nb; k += 64) tasks[m.task_off + base + k] = wbuf[k];
+    wave_sync_lds();
+    nb = 0;
+  };
+  const int g_lo = cls_grp_off[m.bra_class], g_hi = cls_grp_off[m.bra_class + 1];
+  for (int g = g_lo; g < g_hi; ++g) {
+  const int grp_end = (g + 1 < g_hi) ? grp[g + 1] : m.nbra;
+  if (grp_end <= i_begin) continue;
+  for (int i0 = max(grp[g], i_begin); i0 < grp_end; i0 += 64) {
+    const double Qi0 = pairQ[m.bra_start + i0];
+    if (nglob * Qk * Qi0 * (1.0 + 1e-12) < tau) break;
+    const int i = i0 + lane;
     bool pass = false;
-    if (j < ket_end) {
-      const int Kidx = m.ket_start + j;
-      const double Qk = pairQ[Kidx];
-      const PairInfo pk = pairs[Kidx];
-      const long canonK = (long)pk.s1 * (pk.s1 + 1) / 2 + pk.s2;
+    if (i < grp_end) {
+      const int P = m.bra_start + i;
+      const double Qp = pairQ[P];
+      const int2 sp = s12[P];
+      const long canonP = (long)sp.x * (sp.x + 1) / 2 + sp.y;
       int s1, s2, s3, s4;
       double Q12, Q34;
-      if (canonP >= canonK) { s1 = pb.s1; s2 = pb.s2; s3 = pk.s1; s4 = pk.s2; Q12 = Qp; Q34 = Qk; }
-      else                  { s1 = pk.s1; s2 = pk.s2; s3 = pb.s1; s4 = pb.s2; Q12 = Qk; Q34 = Qp; }
+      if (canonP >= canonK) { s1 = sp.x; s2 = sp.y; s3 = sk.x; s4 = sk.y; Q12 = Qp; Q34 = Qk; }
+      else                  { s1 = sk.x; s2 = sk.y; s3 = sp.x; s4 = sp.y; Q12 = Qk; Q34 = Qp; }
       const double n14 = Nmax[s1 * nshell + s4], n24 = Nmax[s2 * nshell + s4];
       const double n34 = Nmax[s3 * nshell + s4], n13 = Nmax[s1 * nshell + s3];
       const double n23 = Nmax[s2 * nshell + s3], n12 = Nmax[s1 * nshell + s2];
       const double mx = fmax(n14, fmax(n24, fmax(n34, fmax(n13, fmax(n23, n12)))));
-      const double bound = __dmul_rn(__dmul_rn(mx, Q12), Q34);
-      pass = !(bound < tau);
+      pass = !(__dmul_rn(__dmul_rn(mx, Q12), Q34) < tau);
     }
     const unsigned long long mask = __ballot(pass);
-    if (mask) {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&counts[cp], (unsigned long long)__popcll(mask));
-      base = __shfl(base, 0);
-      if (pass) {
-        const unsigned long long below = mask & ((1ull << lane) - 1ull);
-        tasks[m.task_off + base + __popcll(below)] = make_uint2((unsigned)P, (unsigned)(m.ket_start + j));
-      }
-    }
+    const int cnt = __popcll(mask);
+    if (nb + cnt > SCREEN_BUF) flush();
+    if (pass) wbuf[nb + __popcll(mask & ((1ull << lane) - 1ull))] = make_uint2((unsigned)(m.bra_start + i), (unsigned)K);
+    nb += cnt;
   }
+  }
+  if (nb > 0) flush();
 }
 
 // ---------------------------------------------------------------------------------------
@@ -451,6 +475,7 @@ struct cq_ctx {
   std::vector<unsigned long long> last_counts;
   // profiling: events around every class-pair kernel of every contraction
   bool profiling = false;
+  std::vector<int> launch_order;
   std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof_events;
   std::vector<double> prof_ms;
   std::vector<long long> prof_launches;
@@ -476,6 +501,13 @@ struct cq_ctx {
   DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start, d_flags;
   DevBuf<double> d_tco, d_boys, d_pairQ, d_Nmax, d_work, d_mats;
   DevBuf<PairInfo> d_pairs;
+  DevBuf<int2> d_s12;
+  std::vector<int> grp, cls_grp_off;
+  DevBuf<int> d_grp, d_cls_grp_off;
+  int n_streams = 4;
+  static constexpr int kAux = 4;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kAux] = {};
   DevBuf<PrimPair> d_prims;
   DevBuf<CPMeta> d_cps;
   DevBuf<uint2> d_tasks;
@@ -489,6 +521,12 @@ struct cq_ctx {
     d_cps.release(); d_tasks.release(); d_counts.release(); d_nglob.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
+    for (int k = 0; k < kAux; ++k) {
+      if (aux[k]) (void)hipStreamDestroy(aux[k]);
+      if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    d_s12.release(); d_grp.release(); d_cls_grp_off.release();
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -514,6 +552,11 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
                    const double* coef, const double* origin) {
   HIPCHK(hipGetDevice(&c->device));
   HIPCHK(hipStreamCreate(&c->stream));
+  for (int k = 0; k < cq_ctx::kAux; ++k) {
+    HIPCHK(hipStreamCreate(&c->aux[k]));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   c->nshell = nshell;
   c->shells.resize(nshell);
   std::vector<int> poff(nshell + 1, 0);
@@ -650,12 +693,17 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   }
   c->pairQ.resize(npair);
   HIPCHK(hipMemcpy(c->pairQ.data(), c->d_pairQ.p, npair * sizeof(double), hipMemcpyDeviceToHost));
-  // sort within class by Schwarz bound, descending (ties: canonical order)
+  // sort within class by primitive-pair count (descending), then Schwarz bound (descending):
+  // equal counts side by side keep the lanes of a wavefront in step, and inside each count
+  // group the bound order lets screening stop early
   std::vector<int> order(npair);
   std::iota(order.begin(), order.end(), 0);
   for (int X = 0; X < c->npc; ++X)
-    std::stable_sort(order.begin() + c->class_start[X], order.begin() + c->class_start[X + 1],
-                     [&](int x, int y) { return c->pairQ[x] > c->pairQ[y]; });
+    std::stable_sort(order.begin() + c->class_start[X], order.begin() + c->class_start[X + 1], [&](int x, int y) {
+      const size_t nx = hp[x].prim.size(), ny = hp[y].prim.size();
+      if (nx != ny) return nx > ny;
+      return c->pairQ[x] > c->pairQ[y];
+    });
   std::vector<HostPair> hp2;
   std::vector<double> q2;
   hp2.reserve(npair);
@@ -666,6 +714,27 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_pairs.upload(c->pairs, c->stream);
   c->d_prims.upload(c->prims, c->stream);
   c->d_pairQ.upload(c->pairQ, c->stream);
+  // primitive-count groups per class: grp[cls_grp_off[X] .. cls_grp_off[X+1]] are the group
+  // starts (class-local), closed by the class size
+  c->cls_grp_off.assign(c->npc + 1, 0);
+  c->grp.clear();
+  for (int X = 0; X < c->npc; ++X) {
+    c->cls_grp_off[X] = (int)c->grp.size();
+    for (int k = c->class_start[X]; k < c->class_start[X + 1]; ++k) {
+      const int np_ = c->pairs[k].p1 - c->pairs[k].p0;
+      if (k == c->class_start[X] || np_ != c->pairs[k - 1].p1 - c->pairs[k - 1].p0)
+        c->grp.push_back(k - c->class_start[X]);
+    }
+  }
+  c->cls_grp_off[c->npc] = (int)c->grp.size();
+  c->d_grp.upload(c->grp, c->stream);
+  c->d_cls_grp_off.upload(c->cls_grp_off, c->stream);
+  {
+    std::vector<int2> s12(npair);
+    for (int k = 0; k < npair; ++k) s12[k] = make_int2(c->pairs[k].s1, c->pairs[k].s2);
+    c->d_s12.upload(s12, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   c->Q.assign((size_t)nshell * nshell, 0.0);
   for (int k = 0; k < npair; ++k) {
     const PairInfo& pi_ = c->pairs[k];
@@ -685,12 +754,13 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     m.ket_start = c->class_start[ck.ki];
     m.nket = c->class_start[ck.ki + 1] - c->class_start[ck.ki];
     m.same = ck.i == ck.j;
+    m.bra_class = ck.bi;
     m.task_off = off;
     const long long cap = m.same ? (long long)m.nbra * (m.nbra + 1) / 2 : (long long)m.nbra * m.nket;
     off += cap;
     c->cps.push_back(m);
     c->item_start.push_back(items);
-    items += m.nbra;
+    items += m.nket;
   }
   c->nitem = items;
   c->task_capacity = off;
@@ -702,6 +772,10 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_Nmax.ensure((size_t)nshell * nshell);
   c->last_counts.assign(kNumClassPairs, 0);
   c->prof_events.assign(kNumClassPairs, {});
+  c->launch_order.resize(kNumClassPairs);
+  std::iota(c->launch_order.begin(), c->launch_order.end(), 0);
+  std::stable_sort(c->launch_order.begin(), c->launch_order.end(),
+                   [](int x, int y) { return kClassKernels[x].L < kClassKernels[y].L; });
   c->prof_ms.assign(kNumClassPairs, 0.0);
   c->prof_launches.assign(kNumClassPairs, 0);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -770,7 +844,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   HIPCHK(hipMemsetAsync(c->d_counts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
   if (c->nitem > 0) {
     hipLaunchKernelGGL(screen_kernel, dim3(grid_for(c->nitem, WG / 64)), dim3(WG), 0, s, c->d_cps.p,
-                       kNumClassPairs, c->d_item_start.p, c->nitem, c->d_pairs.p, c->d_pairQ.p, c->d_Nmax.p,
+                       kNumClassPairs, c->d_item_start.p, c->nitem, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
+                       c->d_cls_grp_off.p, c->d_Nmax.p,
                        c->d_nglob.p, ns, c->tau, c->d_tasks.p, c->d_counts.p);
     HIPCHK(hipGetLastError());
   }
@@ -790,27 +865,39 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       jk.do_k[g] = slots[s0 + g].do_k;
     }
     jk.nonsym = c->d_flags.p + s0;
-    for (int k = 0; k < kNumClassPairs; ++k) {
+    // class-pair kernels spread over the auxiliary streams (highest angular momentum first:
+    // those are the latency-bound ones), joined back into s before the finalisation
+    HIPCHK(hipEventRecord(c->ev_fork, s));
+    for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
+    int launched = 0;
+    for (int kk = kNumClassPairs - 1; kk >= 0; --kk) {
+      const int k = c->launch_order[kk];
       const ClassKernels& ck = kClassKernels[k];
       if (c->cps[k].nbra == 0 || c->cps[k].nket == 0) continue;
+      hipStream_t sk = c->aux[launched++ % c->n_streams];
       const uint2* tasks = c->d_tasks.p + c->cps[k].task_off;
       const long long cap = c->cps[k].same ? (long long)c->cps[k].nbra * (c->cps[k].nbra + 1) / 2
                                            : (long long)c->cps[k].nbra * c->cps[k].nket;
-      const int grid = (int)std::min<long long>(persistent_grid(), (cap + WG - 1) / WG);
+      const long long items = ((cap + 63) / 64) * 64 * ck.nchunk;
+      const int grid = (int)std::min<long long>(persistent_grid(), (items + WG - 1) / WG);
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (c->profiling) {
         e0 = c->get_event();
         e1 = c->get_event();
-        HIPCHK(hipEventRecord(e0, s));
+        HIPCHK(hipEventRecord(e0, sk));
       }
-      hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, s, tasks, c->d_counts.p + k, c->d_pairs.p,
+      hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
                          c->d_prims.p, c->d_boys.p, jk);
       HIPCHK(hipGetLastError());
       if (c->profiling) {
-        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventRecord(e1, sk));
         c->prof_events[k].push_back({e0, e1});
       }
+    }
+    for (int a = 0; a < cq_ctx::kAux; ++a) {
+      HIPCHK(hipEventRecord(c->ev_join[a], c->aux[a]));
+      HIPCHK(hipStreamWaitEvent(s, c->ev_join[a], 0));
     }
   }
   // 5. back to the shells' basis and accumulate: AX += 1/2 scale (G + G^T-partner)
@@ -1025,6 +1112,15 @@ int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, 
                          c->d_tco.p, c->d_tco_off.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));   // the host staging vector is released on return
+  });
+}
+
+// Number of streams the class-pair kernels are spread over (1 serialises them, for
+// per-kernel profiling; default 4).
+int cq_set_launch_streams(cq_ctx_t* c, int n) {
+  return guarded(c, [&] {
+    if (n < 1 || n > cq_ctx::kAux) throw std::invalid_argument("stream count out of range");
+    c->n_streams = n;
   });
 }
 

@@ -41,7 +41,7 @@ struct QuartetArgs {
   int bra_p0, bra_p1, ket_p0, ket_p1;
   int a0, b0, c0, d0;
   double deg;
-  bool bra_uniform;
+  bool ket_uniform;          // every lane of the wavefront holds the same ket pair
   bool bra_swap, ket_swap;   // pair held reversed relative to its canonical (s1 >= s2) order
 };
 
@@ -122,9 +122,9 @@ __device__ __forceinline__ void stage_boys(const double* __restrict__ tab, int L
   __syncthreads();
 }
 
-// Add a J_ab contribution; when the whole wavefront shares one bra pair the 64 values are
-// summed across lanes first and one lane issues the atomic.
-__device__ __forceinline__ void jab_add(double* p, double v, bool uniform) {
+// Atomic add of a value whose target is the same on every lane when `uniform` (wavefront-
+// uniform flag): the 64 values are summed across lanes first and one lane issues the atomic.
+__device__ __forceinline__ void wave_add(double* p, double v, bool uniform) {
   if (uniform) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);

@@ -22,28 +22,69 @@ __device__ __forceinline__ void make_quartet(const uint2* __restrict__ tasks, lo
   q.a0 = B.a0; q.b0 = B.b0; q.c0 = K.a0; q.d0 = K.b0;
   const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
   q.deg = active ? deg : 0.0;
-  const unsigned int b0 = __shfl((int)tk.x, 0);
-  q.bra_uniform = (__ballot(tk.x == b0) == ~0ull);
+  const unsigned int k0 = __shfl((int)tk.y, 0);
+  q.ket_uniform = (__ballot(tk.y == k0) == ~0ull);
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
 }
 
+// Quartet of one lane's bra pair B with the wavefront-uniform ket pair kk (kk must come from
+// readfirstlane so the ket data is loaded through the scalar path); lanes that do not own a
+// task of this ket get an empty bra range and degeneracy 0.
+__device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsigned bra_idx, unsigned kk, bool mine,
+                                                         const PairInfo* __restrict__ pairs,
+                                                         const PrimPair* __restrict__ prims, QuartetArgs& q) {
+  const PairInfo K = pairs[kk];
+  q.prim = prims;
+  q.bra_p0 = B.p0;
+  q.bra_p1 = mine ? B.p1 : B.p0;
+  q.ket_p0 = K.p0;
+  q.ket_p1 = K.p1;
+  q.a0 = B.a0; q.b0 = B.b0; q.c0 = K.a0; q.d0 = K.b0;
+  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (bra_idx == kk ? 1.0 : 2.0);
+  q.deg = mine ? deg : 0.0;
+  q.ket_uniform = true;
+  q.bra_swap = B.a != B.s1;
+  q.ket_swap = K.a != K.s1;
+}
+
+// JK kernel: work items are (chunk, 64-task block) pairs, one chunk per wavefront, so classes
+// whose ket components are split into independent chunks run them in parallel.
 #define DEFINE_CLASS_KERNELS(I, J, BI, KI, NAME, LTOT)                                          \
   template <int NSLOT>                                                                          \
   __global__ void __launch_bounds__(WG) eri_jk_##NAME(                                          \
       const uint2* __restrict__ tasks, const unsigned long long* __restrict__ count_ptr,        \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                   \
       const double* __restrict__ boys, JKArgs jk) {                                             \
+    constexpr int NCH = kChunks_##NAME;                                                         \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                        \
     const long count = (long)*count_ptr;                                                        \
-    if ((long)blockIdx.x * WG >= count) return;                                                 \
+    const long nitems = ((count + 63) / 64) * 64 * NCH;                                         \
+    if ((long)blockIdx.x * WG >= nitems) return;                                                \
     stage_boys(boys, LTOT, boys_lds);                                                           \
-    for (long base = (long)blockIdx.x * WG; base < count; base += (long)gridDim.x * WG) {      \
-      const long wbase = base + (threadIdx.x & ~63);                                            \
-      if (wbase >= count) break;                                                                \
-      QuartetArgs q;                                                                            \
-      make_quartet(tasks, base + threadIdx.x, count, wbase, pairs, prims, q);                   \
-      quartet_jk_##NAME<NSLOT>(q, jk, boys_lds);                                                \
+    for (long base = (long)blockIdx.x * WG; base < nitems; base += (long)gridDim.x * WG) {     \
+      const long witem = (base + (threadIdx.x & ~63)) / 64;                                     \
+      const int chunk = (int)(witem % NCH);                                                     \
+      const long tbase = (witem / NCH) * 64;                                                    \
+      if (tbase >= count) break;                                                                \
+      const long t = tbase + (threadIdx.x & 63);                                                \
+      const bool active = t < count;                                                            \
+      const uint2 tk = tasks[active ? t : tbase];                                               \
+      const PairInfo B = pairs[tk.x];                                                           \
+      bool pending = active;                                                                    \
+      /* waterfall over the distinct ket pairs of the wavefront: each pass has one uniform ket \
+         (scalar loads of its primitive data), lanes of other kets contribute nothing */        \
+      while (true) {                                                                            \
+        const unsigned long long pm = __ballot(pending);                                        \
+        if (pm == 0ull) break;                                                                  \
+        const int leader = __ffsll((long long)pm) - 1;                                          \
+        const unsigned kk = __builtin_amdgcn_readfirstlane(__shfl((int)tk.y, leader));          \
+        const bool mine = pending && tk.y == kk;                                                \
+        QuartetArgs q;                                                                          \
+        make_quartet_uniform_ket(B, tk.x, kk, mine, pairs, prims, q);                           \
+        quartet_jk_##NAME<NSLOT>(q, jk, boys_lds, chunk);                                       \
+        pending = pending && !mine;                                                             \
+      }                                                                                         \
     }                                                                                           \
   }                                                                                             \
   __global__ void __launch_bounds__(WG) eri_block_##NAME(                                       \
@@ -68,7 +109,7 @@ typedef void (*JKKernel)(const uint2*, const unsigned long long*, const PairInfo
 typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int);
 
 struct ClassKernels {
-  int i, j, bi, ki, L;
+  int i, j, bi, ki, L, nchunk;
   const char* name;
   JKKernel jk1, jk2;
   BlockKernel block;

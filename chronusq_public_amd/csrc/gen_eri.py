@@ -206,7 +206,7 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
     em(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}) kind={kind} chunks={len(chunks)} persist={persist}")
     if kind == "jk":
         em("template <int NSLOT>")
-        em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds) {{")
+        em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds, int chunk) {{")
     else:
         em(f"__device__ __forceinline__ void quartet_block_{name}(const QuartetArgs& q, double* __restrict__ out, const double* __restrict__ boys_lds) {{")
     em.ind = 1
@@ -221,7 +221,11 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
             for idd in range(ND):
                 em(f"double kbd_{ib}_{idd}[NSLOT] = {{}};")
     for ci, ks in enumerate(chunks):
-        em(f"{{ // chunk {ci}: (c,d) components {ks}")
+        if kind == "jk" and not persist:
+            # chunks without shared accumulators are independent: one per wavefront
+            em(f"if (chunk == {ci}) {{ // chunk {ci}: (c,d) components {ks}")
+        else:
+            em(f"{{ // chunk {ci}: (c,d) components {ks}")
         em.ind += 1
         for ia in range(NA):
             for ib in range(NB):
@@ -322,6 +326,8 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
         gen_digest_final(em, NA, NB, ND)
     em.ind = 0
     em("}")
+    if kind == "jk":
+        em(f"constexpr int kChunks_{name} = {1 if persist else len(chunks)};")
     return "\n".join(em.lines) + "\n"
 
 
@@ -373,6 +379,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
     em("const double dg = q.deg;")
     em("const double hk = 0.5 * q.deg;")
     em("const bool ns = jk.nonsym[s] != 0;")
+    em("const bool live = q.deg != 0.0;   // lanes without a quartet in this pass skip their atomics")
     em("if (jk.do_j[s]) {")
     em.ind += 1
     if not persist:
@@ -387,11 +394,11 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                 em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('xba', ia, ib)}];")
                 tgt = f"jab_{ia}_{ib}[s]" if persist else f"jab_{ia}_{ib}"
                 em(f"  {tgt} += o_{ia}_{ib}_{ic}_{idd} * x43;")
-        em(f"  atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
+        em(f"  wave_add(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd, q.ket_uniform); }}")
     if not persist:
         for ia in range(NA):
             for ib in range(NB):
-                em(f"jab_add(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}, q.bra_uniform);")
+                em(f"if (live) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib});")
     em("if (ns) {")
     em.ind += 1
     for (ic, idd) in ks:
@@ -402,8 +409,8 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                 em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('tab', ia, ib)}];")
         for ia in range(NA):
             for ib in range(NB):
-                em(f"  atomicAdd(&jk.GJ2[s][{_ab('tab', ia, ib)}], o_{ia}_{ib}_{ic}_{idd} * x34);")
-        em(f"  atomicAdd(&jk.GJ2[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
+                em(f"  if (live) atomicAdd(&jk.GJ2[s][{_ab('tab', ia, ib)}], o_{ia}_{ib}_{ic}_{idd} * x34);")
+        em(f"  wave_add(&jk.GJ2[s][{_cd('tcd', ic, idd)}], dg * jcd, q.ket_uniform); }}")
     em.ind -= 1
     em("}")
     em.ind -= 1
@@ -425,7 +432,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (jc, idd) in ks:
                         if jc == ic:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'd0', idd, trans)};")
-                em(f"  atomicAdd(&{G}[(q.a0 + {ia}) * N + q.c0 + {ic}], hk * acc); }}")
+                em(f"  if (live) atomicAdd(&{G}[(q.a0 + {ia}) * N + q.c0 + {ic}], hk * acc); }}")
         for ib in range(NB):
             for ic in cs:
                 em("{ double acc = 0.0;")
@@ -433,7 +440,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (jc, idd) in ks:
                         if jc == ic:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'd0', idd, trans)};")
-                em(f"  atomicAdd(&{G}[(q.b0 + {ib}) * N + q.c0 + {ic}], hk * acc); }}")
+                em(f"  if (live) atomicAdd(&{G}[(q.b0 + {ib}) * N + q.c0 + {ic}], hk * acc); }}")
         if not with_ad_bd:
             return
         # K(a,d) += X(b,c) ; K(b,d) += X(a,c)   (partial over the chunk's c)
@@ -444,7 +451,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (ic, jd) in ks:
                         if jd == idd:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'c0', ic, trans)};")
-                em(f"  atomicAdd(&{G}[(q.a0 + {ia}) * N + q.d0 + {idd}], hk * acc); }}")
+                em(f"  if (live) atomicAdd(&{G}[(q.a0 + {ia}) * N + q.d0 + {idd}], hk * acc); }}")
         for ib in range(NB):
             for idd in ds:
                 em("{ double acc = 0.0;")
@@ -452,7 +459,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (ic, jd) in ks:
                         if jd == idd:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'c0', ic, trans)};")
-                em(f"  atomicAdd(&{G}[(q.b0 + {ib}) * N + q.d0 + {idd}], hk * acc); }}")
+                em(f"  if (live) atomicAdd(&{G}[(q.b0 + {ib}) * N + q.d0 + {idd}], hk * acc); }}")
 
     kupdates("jk.GK[s]", False, not persist)
     if persist:
@@ -491,12 +498,12 @@ def gen_digest_final(em: Emitter, NA, NB, ND):
     em("const int tab_a = q.bra_swap ? 1 : N, tab_b = q.bra_swap ? N : 1;")
     em("const double dg = q.deg;")
     em("const double hk = 0.5 * q.deg;")
-    em("if (jk.do_j[s]) {")
+    em("if (q.deg != 0.0 && jk.do_j[s]) {")
     for ia in range(NA):
         for ib in range(NB):
-            em(f"  jab_add(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}[s], q.bra_uniform);")
+            em(f"  atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}[s]);")
     em("}")
-    em("if (jk.do_k[s]) {")
+    em("if (q.deg != 0.0 && jk.do_k[s]) {")
     for ia in range(NA):
         for idd in range(ND):
             em(f"  atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
@@ -546,7 +553,7 @@ def main(out_dir: str, maxl: int = 2):
             src.append(gen_class(LA, LB, LC, LD, "block"))
             src.append(f"DEFINE_CLASS_KERNELS({i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {name}, {L})")
             src.append(f"void cq_register_{name}(ClassKernels& ck) {{")
-            src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, \"{name}\", "
+            src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, kChunks_{name}, \"{name}\", "
                        f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}}};")
             src.append("}")
             _write_if_changed(os.path.join(out_dir, f"eri_{name}.hip"), "\n".join(src) + "\n")

@@ -200,8 +200,11 @@ class SingleSlater:
                     Bm[n, n] = 0.0
                     rhs = torch.zeros(n + 1, dtype=torch.float64, device=self.dev)
                     rhs[n] = -1.0
-                    c = torch.linalg.solve(Bm, rhs)[:n]
-                    if bool(torch.isfinite(c).all()):
+                    # a singular system (vanishing commutators) keeps the plain step
+                    # (singleslater_diis.h:60-64)
+                    c, info = torch.linalg.solve_ex(Bm, rhs)
+                    c = c[:n]
+                    if int(info) == 0 and bool(torch.isfinite(c).all()):
                         Fs = tuple(sum(c[i] * hist_F[i][k] for i in range(n)) for k in range(len(Fs)))
             if damp > 0.0 and Fo_prev is not None:
                 Fs = tuple((1 - damp) * f + damp * fp for f, fp in zip(Fs, Fo_prev))

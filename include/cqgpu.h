@@ -94,6 +94,10 @@ int cq_get_quartet_counts(cq_ctx_t* ctx, long long* counts, int max_counts, int*
 int cq_num_class_pairs(void);
 int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld);
 
+/* Number of HIP streams (1..4, default 4) the class-pair kernels of one contraction are spread
+ * over; 1 runs them one after another (for per-kernel timing). */
+int cq_set_launch_streams(cq_ctx_t* ctx, int n);
+
 /* Profiling (no reference counterpart beyond its wall-clock G[P] timer,
  * include/singleslater/singleslater_fock.h:116-119): HIP events around every class-pair
  * kernel; accumulated milliseconds and launch counts per class pair. */

@@ -77,5 +77,5 @@ def test_device_uhf_golden_energies():
     for test, rec in d["tests"].items():
         sysd = d["systems"][rec["system"]]
         mol = M.Molecule.from_angstrom(sysd["symbols"], sysd["angstrom"], 0, sysd["mult"])
-        r = SingleSlater(mol, rec["basis"], ref="UHF", tau=1e-14, guess="CORE").SCF3()
+        r = SingleSlater(mol, rec["basis"], ref="UHF", tau=1e-14, guess="SAD").SCF3()
         assert abs(r.energy - rec["E"]) < d["tolerance"], (test, r.energy, rec["E"])
