@@ -43,6 +43,8 @@
 #include "_gen/eri_registry.inc"
 
 static ClassKernels kClassKernels[CQ_NUM_CLASS_PAIRS];
+// dynamic LDS of the segment kernels (static LDS: Boys table, 26 KB)
+static const size_t kMaxDynLds = 36 * 1024;
 static const int kNumClassPairs = CQ_NUM_CLASS_PAIRS;
 static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 
@@ -52,6 +54,7 @@ static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 struct CPMeta {
   int bra_start, nbra, ket_start, nket, same, bra_class;
   long long task_off;
+  long long seg_off;
 };
 
 // One wavefront per (class pair, ket pair); lanes sweep the bra pairs (sorted by Schwarz
@@ -69,7 +72,7 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     const int2* __restrict__ s12, const double* __restrict__ pairQ, const int* __restrict__ grp,
     const int* __restrict__ cls_grp_off, const double* __restrict__ Nmax,
     const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, uint2* __restrict__ tasks,
-    unsigned long long* __restrict__ counts) {
+    unsigned long long* __restrict__ counts, Segment* __restrict__ segs, unsigned* __restrict__ nseg) {
   __shared__ uint2 buf[WG / 64][SCREEN_BUF];
   const int wid = threadIdx.x >> 6;
   const int wave = blockIdx.x * (WG / 64) + wid;
@@ -90,7 +93,11 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   auto flush = [&]() {
     wave_sync_lds();
     unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(&counts[cp], (unsigned long long)nb);
+    if (lane == 0) {
+      base = atomicAdd(&counts[cp], (unsigned long long)nb);
+      const unsigned si = atomicAdd(&nseg[cp], 1u);
+      segs[m.seg_off + si] = Segment{(int)base, nb, K, 0};
+    }
     base = __shfl(base, 0);
     for (int k = lane; k < nb; k += 64) tasks[m.task_off + base + k] = wbuf[k];
     wave_sync_lds();
@@ -511,6 +518,9 @@ struct cq_ctx {
   DevBuf<PrimPair> d_prims;
   DevBuf<CPMeta> d_cps;
   DevBuf<uint2> d_tasks;
+  DevBuf<Segment> d_segs;
+  DevBuf<unsigned> d_nseg, d_work_q;
+  bool use_lds = true;
   DevBuf<unsigned long long> d_counts, d_nglob;
   std::string err;
 
@@ -518,7 +528,7 @@ struct cq_ctx {
     d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
     d_item_start.release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
     d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
-    d_cps.release(); d_tasks.release(); d_counts.release(); d_nglob.release();
+    d_cps.release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_nglob.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < kAux; ++k) {
@@ -744,7 +754,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   // class-pair metadata and task capacity
   c->cps.clear();
   c->item_start.clear();
-  long long off = 0;
+  long long off = 0, seg_total = 0;
   int items = 0;
   for (int k = 0; k < kNumClassPairs; ++k) {
     const ClassKernels& ck = kClassKernels[k];
@@ -758,6 +768,9 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     m.task_off = off;
     const long long cap = m.same ? (long long)m.nbra * (m.nbra + 1) / 2 : (long long)m.nbra * m.nket;
     off += cap;
+    // each ket's survivors are flushed in pieces of at most SCREEN_BUF tasks
+    m.seg_off = seg_total;
+    seg_total += m.nket + cap / SCREEN_BUF + 1;
     c->cps.push_back(m);
     c->item_start.push_back(items);
     items += m.nket;
@@ -767,6 +780,9 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_cps.upload(c->cps, c->stream);
   c->d_item_start.upload(c->item_start, c->stream);
   c->d_tasks.ensure((size_t)std::max<long long>(1, off));
+  c->d_segs.ensure((size_t)std::max<long long>(1, seg_total));
+  c->d_nseg.ensure(kNumClassPairs);
+  c->d_work_q.ensure(kNumClassPairs);
   c->d_counts.ensure(kNumClassPairs);
   c->d_nglob.ensure(1);
   c->d_Nmax.ensure((size_t)nshell * nshell);
@@ -842,11 +858,13 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   }
   // 3. screening into compacted task lists
   HIPCHK(hipMemsetAsync(c->d_counts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->d_nseg.p, 0, kNumClassPairs * sizeof(unsigned), s));
+  HIPCHK(hipMemsetAsync(c->d_work_q.p, 0, kNumClassPairs * sizeof(unsigned), s));
   if (c->nitem > 0) {
     hipLaunchKernelGGL(screen_kernel, dim3(grid_for(c->nitem, WG / 64)), dim3(WG), 0, s, c->d_cps.p,
                        kNumClassPairs, c->d_item_start.p, c->nitem, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
                        c->d_cls_grp_off.p, c->d_Nmax.p,
-                       c->d_nglob.p, ns, c->tau, c->d_tasks.p, c->d_counts.p);
+                       c->d_nglob.p, ns, c->tau, c->d_tasks.p, c->d_counts.p, c->d_segs.p, c->d_nseg.p);
     HIPCHK(hipGetLastError());
   }
   // 4. ERI + J/K digestion, class pair by class pair, two density slots per pass
@@ -881,14 +899,25 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const long long items = ((cap + 63) / 64) * 64 * ck.nchunk;
       const int grid = (int)std::min<long long>(persistent_grid(), (items + WG - 1) / WG);
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
+      // one density slot and LDS room for the K columns: segment kernel (LDS accumulation)
+      const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->ncart + (size_t)ck.nc * ck.nd) * sizeof(double);
+      const bool seg_path = c->use_lds && ng == 1 && lds_bytes <= kMaxDynLds;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (c->profiling) {
         e0 = c->get_event();
         e1 = c->get_event();
         HIPCHK(hipEventRecord(e0, sk));
       }
-      hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
-                         c->d_prims.p, c->d_boys.p, jk);
+      if (seg_path) {
+        const long long seg_cap = c->cps[k].nket + cap / SCREEN_BUF + 1;
+        const int sgrid = (int)std::min<long long>(persistent_grid(), seg_cap * ck.nchunk);
+        hipLaunchKernelGGL(ck.jkseg, dim3(std::max(1, sgrid)), dim3(WG), lds_bytes, sk, tasks,
+                           c->d_segs.p + c->cps[k].seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
+                           c->d_prims.p, c->d_boys.p, jk);
+      } else {
+        hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
+                           c->d_prims.p, c->d_boys.p, jk);
+      }
       HIPCHK(hipGetLastError());
       if (c->profiling) {
         HIPCHK(hipEventRecord(e1, sk));
@@ -1113,6 +1142,12 @@ int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, 
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));   // the host staging vector is released on return
   });
+}
+
+// Selects the J/K digestion: 1 (default) accumulates K columns and J(3,4) of each ket pair in
+// LDS and flushes once per work item; 0 issues every update as an HBM atomic.
+int cq_set_lds_digestion(cq_ctx_t* c, int on) {
+  return guarded(c, [&] { c->use_lds = on != 0; });
 }
 
 // Number of streams the class-pair kernels are spread over (1 serialises them, for

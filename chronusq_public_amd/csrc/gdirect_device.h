@@ -122,6 +122,19 @@ __device__ __forceinline__ void stage_boys(const double* __restrict__ tab, int L
   __syncthreads();
 }
 
+// Work unit of the segment kernels: `count` tasks of one ket pair starting at `off` in the
+// class pair's task list.
+struct Segment {
+  int off, count, ket, pad;
+};
+
+// Wavefront-reduced add into LDS (every lane targets the same LDS word).
+__device__ __forceinline__ void wave_add_lds(double* p, double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(p, v);
+}
+
 // Atomic add of a value whose target is the same on every lane when `uniform` (wavefront-
 // uniform flag): the 64 values are summed across lanes first and one lane issues the atomic.
 __device__ __forceinline__ void wave_add(double* p, double v, bool uniform) {

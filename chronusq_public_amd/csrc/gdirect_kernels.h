@@ -82,11 +82,73 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
         const bool mine = pending && tk.y == kk;                                                \
         QuartetArgs q;                                                                          \
         make_quartet_uniform_ket(B, tk.x, kk, mine, pairs, prims, q);                           \
-        quartet_jk_##NAME<NSLOT>(q, jk, boys_lds, chunk);                                       \
+        quartet_jk_##NAME<NSLOT, false>(q, jk, boys_lds, chunk, nullptr, nullptr);                                       \
         pending = pending && !mine;                                                             \
       }                                                                                         \
     }                                                                                           \
   }                                                                                             \
+  /* Segment kernel (one density slot): each workgroup takes (segment, chunk) items from a  \
+     queue; the segment's ket is uniform, K columns c/d and J(3,4) accumulate in LDS and are \
+     flushed to HBM once per item. */                                                        \
+  __global__ void __launch_bounds__(WG) eri_jkseg_##NAME(                                     \
+      const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                      \
+      const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                     \
+      const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                 \
+      const double* __restrict__ boys, JKArgs jk) {                                           \
+    constexpr int NCH = kChunks_##NAME, NC = kNC_##NAME, ND = kND_##NAME, NCOL = NC + ND;     \
+    __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                      \
+    __shared__ int s_item;                                                                    \
+    extern __shared__ double dyn_lds[];                                                       \
+    const long nitems = (long)(*nseg_ptr) * NCH;                                              \
+    if (nitems == 0) return;                                                                  \
+    stage_boys(boys, LTOT, boys_lds);                                                         \
+    const int N = jk.ncart;                                                                   \
+    double* colK = dyn_lds;                                                                   \
+    double* jcdL = dyn_lds + NCOL * N;                                                        \
+    for (int k = threadIdx.x; k < NCOL * N + NC * ND; k += WG) dyn_lds[k] = 0.0;             \
+    const int wofs = threadIdx.x & ~63;                                                       \
+    while (true) {                                                                            \
+      __syncthreads();                                                                        \
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(work, 1u);                                \
+      __syncthreads();                                                                        \
+      const long item = s_item;                                                               \
+      if (item >= nitems) break;                                                              \
+      const Segment sg = segs[item / NCH];                                                    \
+      const int chunk = (int)(item % NCH);                                                    \
+      const unsigned kk = __builtin_amdgcn_readfirstlane((unsigned)sg.ket);                   \
+      for (int t0 = 0; t0 < sg.count; t0 += WG) {                                             \
+        if (t0 + wofs >= sg.count) continue;                                                  \
+        const int t = t0 + threadIdx.x;                                                       \
+        const bool active = t < sg.count;                                                     \
+        const uint2 tk = tasks[sg.off + (active ? t : t0 + wofs)];                            \
+        const PairInfo B = pairs[tk.x];                                                       \
+        QuartetArgs q;                                                                        \
+        make_quartet_uniform_ket(B, tk.x, kk, active, pairs, prims, q);                       \
+        quartet_jk_##NAME<1, true>(q, jk, boys_lds, chunk, colK, jcdL);                       \
+      }                                                                                       \
+      __syncthreads();                                                                        \
+      const PairInfo K = pairs[kk];                                                           \
+      for (int k = threadIdx.x; k < NCOL * N; k += WG) {                                      \
+        const double v = colK[k];                                                             \
+        if (v != 0.0) {                                                                       \
+          const int col = k / N, row = k - col * N;                                           \
+          const int gcol = col < NC ? K.a0 + col : K.b0 + (col - NC);                         \
+          atomicAdd(&jk.GK[0][(long)row * N + gcol], v);                                      \
+          colK[k] = 0.0;                                                                      \
+        }                                                                                     \
+      }                                                                                       \
+      if (threadIdx.x < NC * ND) {                                                            \
+        const double v = jcdL[threadIdx.x];                                                   \
+        if (v != 0.0) {                                                                       \
+          const bool ks = K.a != K.s1;                                                        \
+          const int ic = threadIdx.x / ND, idd = threadIdx.x - (threadIdx.x / ND) * ND;       \
+          const int tcd = ks ? K.b0 * N + K.a0 : K.a0 * N + K.b0;                             \
+          atomicAdd(&jk.GJ[0][tcd + ic * (ks ? 1 : N) + idd * (ks ? N : 1)], v);              \
+          jcdL[threadIdx.x] = 0.0;                                                            \
+        }                                                                                     \
+      }                                                                                       \
+    }                                                                                         \
+  }                                                                                           \
   __global__ void __launch_bounds__(WG) eri_block_##NAME(                                       \
       const uint2* __restrict__ tasks, long count, const PairInfo* __restrict__ pairs,          \
       const PrimPair* __restrict__ prims, const double* __restrict__ boys, double* out,          \
@@ -107,12 +169,16 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
 typedef void (*JKKernel)(const uint2*, const unsigned long long*, const PairInfo*, const PrimPair*,
                          const double*, JKArgs);
 typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int);
+typedef void (*SegKernel)(const uint2*, const Segment*, const unsigned*, unsigned*, const PairInfo*, const PrimPair*,
+                          const double*, JKArgs);
 
 struct ClassKernels {
   int i, j, bi, ki, L, nchunk;
   const char* name;
   JKKernel jk1, jk2;
   BlockKernel block;
+  SegKernel jkseg;
+  int nc, nd;
 };
 
 

@@ -205,8 +205,8 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
     em.ind = 0
     em(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}) kind={kind} chunks={len(chunks)} persist={persist}")
     if kind == "jk":
-        em("template <int NSLOT>")
-        em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds, int chunk) {{")
+        em("template <int NSLOT, bool LDSK>")
+        em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds, int chunk, double* colK, double* jcdL) {{")
     else:
         em(f"__device__ __forceinline__ void quartet_block_{name}(const QuartetArgs& q, double* __restrict__ out, const double* __restrict__ boys_lds) {{")
     em.ind = 1
@@ -319,15 +319,16 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                         idx = ((ia * NB + ib) * NC + ic) * ND + idd
                         em(f"out[{idx}] = o_{ia}_{ib}_{ic}_{idd};")
         else:
-            gen_digest_chunk(em, NA, NB, ks, persist)
+            gen_digest_chunk(em, NA, NB, NC, ND, ks, persist)
         em.ind -= 1
         em("}")
     if kind == "jk" and persist:
-        gen_digest_final(em, NA, NB, ND)
+        gen_digest_final(em, NA, NB, NC, ND)
     em.ind = 0
     em("}")
     if kind == "jk":
         em(f"constexpr int kChunks_{name} = {1 if persist else len(chunks)};")
+        em(f"constexpr int kNC_{name} = {NC}, kND_{name} = {ND};")
     return "\n".join(em.lines) + "\n"
 
 
@@ -360,7 +361,7 @@ def _cd(form, ic, idd):
     return f"{form} + {ic} * {form}_c + {idd} * {form}_d"
 
 
-def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
+def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
     """Digest one chunk of (c,d) components for every density slot.
 
     Coulomb (forward, canonical in-pair order): G(1,2) += X(4,3) v, G(3,4) += X(2,1) v.
@@ -394,7 +395,8 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                 em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('xba', ia, ib)}];")
                 tgt = f"jab_{ia}_{ib}[s]" if persist else f"jab_{ia}_{ib}"
                 em(f"  {tgt} += o_{ia}_{ib}_{ic}_{idd} * x43;")
-        em(f"  wave_add(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd, q.ket_uniform); }}")
+        em(f"  if constexpr (LDSK) wave_add_lds(&jcdL[s * {NC * ND} + {ic * ND + idd}], dg * jcd);")
+        em(f"  else wave_add(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd, q.ket_uniform); }}")
     if not persist:
         for ia in range(NA):
             for ib in range(NB):
@@ -423,6 +425,18 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
             return f"D[(q.{c0} + {ci}) * N + q.{r0} + {ri}]"
         return f"D[(q.{r0} + {ri}) * N + q.{c0} + {ci}]"
 
+    NCOL = NC + ND
+
+    def kadd(G, trans, row0, ri, colkind, ci, val):
+        """One K update: row (row0 + ri), column c0 + ci (colkind 'c') or d0 + ci ('d')."""
+        col0 = "q.c0" if colkind == "c" else "q.d0"
+        glob = f"atomicAdd(&{G}[(q.{row0} + {ri}) * N + {col0} + {ci}], {val})"
+        if trans:
+            return f"if (live) {glob};"
+        lcol = ci if colkind == "c" else NC + ci
+        lds = f"atomicAdd(&colK[(s * {NCOL} + {lcol}) * N + q.{row0} + {ri}], {val})"
+        return f"if (live) {{ if constexpr (LDSK) {lds}; else {glob}; }}"
+
     def kupdates(G, trans, with_ad_bd):
         # K(a,c) += X(b,d) ; K(b,c) += X(a,d)   (partial over the chunk's d)
         for ia in range(NA):
@@ -432,7 +446,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (jc, idd) in ks:
                         if jc == ic:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'd0', idd, trans)};")
-                em(f"  if (live) atomicAdd(&{G}[(q.a0 + {ia}) * N + q.c0 + {ic}], hk * acc); }}")
+                em(f"  {kadd(G, trans, 'a0', ia, 'c', ic, 'hk * acc')} }}")
         for ib in range(NB):
             for ic in cs:
                 em("{ double acc = 0.0;")
@@ -440,7 +454,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (jc, idd) in ks:
                         if jc == ic:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'd0', idd, trans)};")
-                em(f"  if (live) atomicAdd(&{G}[(q.b0 + {ib}) * N + q.c0 + {ic}], hk * acc); }}")
+                em(f"  {kadd(G, trans, 'b0', ib, 'c', ic, 'hk * acc')} }}")
         if not with_ad_bd:
             return
         # K(a,d) += X(b,c) ; K(b,d) += X(a,c)   (partial over the chunk's c)
@@ -451,7 +465,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (ic, jd) in ks:
                         if jd == idd:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('b0', ib, 'c0', ic, trans)};")
-                em(f"  if (live) atomicAdd(&{G}[(q.a0 + {ia}) * N + q.d0 + {idd}], hk * acc); }}")
+                em(f"  {kadd(G, trans, 'a0', ia, 'd', idd, 'hk * acc')} }}")
         for ib in range(NB):
             for idd in ds:
                 em("{ double acc = 0.0;")
@@ -459,7 +473,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
                     for (ic, jd) in ks:
                         if jd == idd:
                             em(f"  acc += o_{ia}_{ib}_{ic}_{idd} * {X('a0', ia, 'c0', ic, trans)};")
-                em(f"  if (live) atomicAdd(&{G}[(q.b0 + {ib}) * N + q.d0 + {idd}], hk * acc); }}")
+                em(f"  {kadd(G, trans, 'b0', ib, 'd', idd, 'hk * acc')} }}")
 
     kupdates("jk.GK[s]", False, not persist)
     if persist:
@@ -489,7 +503,8 @@ def gen_digest_chunk(em: Emitter, NA, NB, ks, persist):
     em("}")
 
 
-def gen_digest_final(em: Emitter, NA, NB, ND):
+def gen_digest_final(em: Emitter, NA, NB, NC, ND):
+    NCOL = NC + ND
     em("#pragma unroll")
     em("for (int s = 0; s < NSLOT; ++s) {")
     em.ind += 1
@@ -506,10 +521,12 @@ def gen_digest_final(em: Emitter, NA, NB, ND):
     em("if (q.deg != 0.0 && jk.do_k[s]) {")
     for ia in range(NA):
         for idd in range(ND):
-            em(f"  atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
+            em(f"  if constexpr (LDSK) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.a0 + {ia}], hk * kad_{ia}_{idd}[s]);")
+            em(f"  else atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
     for ib in range(NB):
         for idd in range(ND):
-            em(f"  atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], hk * kbd_{ib}_{idd}[s]);")
+            em(f"  if constexpr (LDSK) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.b0 + {ib}], hk * kbd_{ib}_{idd}[s]);")
+            em(f"  else atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], hk * kbd_{ib}_{idd}[s]);")
     em("}")
     em.ind -= 1
     em("}")
@@ -554,7 +571,7 @@ def main(out_dir: str, maxl: int = 2):
             src.append(f"DEFINE_CLASS_KERNELS({i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {name}, {L})")
             src.append(f"void cq_register_{name}(ClassKernels& ck) {{")
             src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, kChunks_{name}, \"{name}\", "
-                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}}};")
+                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}, kNC_{name}, kND_{name}}};")
             src.append("}")
             _write_if_changed(os.path.join(out_dir, f"eri_{name}.hip"), "\n".join(src) + "\n")
             table.append(name)

@@ -194,3 +194,18 @@ def test_full_size_water16_properties():
     got = ao.quartets(np.array(quads))
     for q, g in zip(quads, got):
         assert np.abs(g - ob.quartet(*q)).max() < 1e-12
+
+
+@pytest.mark.parametrize("lds", [0, 1])
+def test_both_digestion_modes_match_oracle(lds):
+    """HBM-atomic and LDS-accumulated digestion against the oracle ((H2O)3 / cc-pVDZ)."""
+    from chronusq_public_amd import _lib
+    mol = M.water_cluster(3, seed=11, box_ang=M.scaled_box(3))
+    bs, ao, ob = _setup(mol, "cc-pvdz")
+    _lib.check(_lib.load().cq_set_lds_digestion(ao._h, lds), ao._h)
+    X = _sym_random(bs.nbasis, 5)
+    Qg = ao.computeSchwartz()
+    J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+    (Jo, Ko), n = _oracle_jk(ob, Qg, [X, X], [0, 1], [1.0, -0.5], 1e-12)
+    assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL
+    assert sum(ao.quartet_counts().values()) == n
