@@ -60,7 +60,8 @@ class SCFResult:
 class SingleSlater:
     def __init__(self, mol: Molecule, basis, ref: str = "RHF", tau: float = 1e-12, guess: str = "SAD",
                  ndiis: int = 10, damp: float = 0.0, ene_tol: float = 1e-10, den_tol: float = 1e-8,
-                 max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None):
+                 max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None,
+                 incremental: bool = False, rebuild_every: int = 8):
         import torch
         self.torch = torch
         self.mol = mol
@@ -68,6 +69,11 @@ class SingleSlater:
         self.ref = ref.upper()
         self.guess = guess.upper()
         self.ndiis, self.damp = ndiis, damp
+        # incremental Fock build: PT(k) = PT(k-1) + G[P(k) - P(k-1)], the difference density
+        # screened by its own block norms; a full build every `rebuild_every` iterations
+        # (the reference carries disabled scaffolding for this, singleslater_scf.h:62-63,
+        # singleslater_scfutils.h:423-564)
+        self.incremental, self.rebuild_every = incremental, rebuild_every
         self.ene_tol, self.den_tol, self.max_iter = ene_tol, den_tol, max_iter
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         t0 = time.perf_counter()
@@ -167,11 +173,19 @@ class SingleSlater:
         res = SCFResult(energy=0.0, iterations=0, converged=False, setup_s=self.setup_s)
         E_old = 0.0
         Fo_prev = None
+        P_prev = PT_prev = None
+        B_hist = None
         damp = self.damp
         for it in range(self.max_iter):
             torch.cuda.synchronize()
             tg = time.perf_counter()
-            PT, PTz = self.formPT(P, Mz if uhf else None)
+            if self.incremental and not uhf and it % self.rebuild_every != 0:
+                dPT, _ = self.formPT(P - P_prev)
+                PT = PT_prev + dPT
+                PTz = None
+            else:
+                PT, PTz = self.formPT(P, Mz if uhf else None)
+            P_prev, PT_prev = P, PT
             torch.cuda.synchronize()
             res.gd_times.append(time.perf_counter() - tg)
             E = self._energy(P, PT, Mz, PTz)
@@ -187,16 +201,27 @@ class SingleSlater:
                 errs = [self.X @ (F @ P @ self.S - self.S @ P @ F) @ self.X]
                 Fs = (F,)
             if self.ndiis > 0:
+                e_new = torch.cat([e.reshape(-1) for e in errs])
                 hist_F.append(Fs)
-                hist_e.append(torch.cat([e.reshape(-1) for e in errs]))
+                hist_e.append(e_new)
+                # one new row of B = <e_i, e_j> per iteration, as a reduction over the N^2
+                # entries (a k x N^2 x k GEMM maps to a single workgroup in rocBLAS)
+                row = (torch.stack(hist_e) * e_new).sum(dim=1)
+                m = 0 if B_hist is None else B_hist.shape[0]
+                B_new = torch.empty((m + 1, m + 1), dtype=torch.float64, device=self.dev)
+                if m:
+                    B_new[:m, :m] = B_hist
+                B_new[m, :] = row
+                B_new[:, m] = row
+                B_hist = B_new
                 if len(hist_F) > self.ndiis:
                     hist_F.pop(0)
                     hist_e.pop(0)
+                    B_hist = B_hist[1:, 1:]
                 n = len(hist_F)
                 if n > 1:
-                    E_mat = torch.stack(hist_e)
                     Bm = -torch.ones((n + 1, n + 1), dtype=torch.float64, device=self.dev)
-                    Bm[:n, :n] = E_mat @ E_mat.T
+                    Bm[:n, :n] = B_hist
                     Bm[n, n] = 0.0
                     rhs = torch.zeros(n + 1, dtype=torch.float64, device=self.dev)
                     rhs[n] = -1.0

@@ -79,3 +79,14 @@ def test_device_uhf_golden_energies():
         mol = M.Molecule.from_angstrom(sysd["symbols"], sysd["angstrom"], 0, sysd["mult"])
         r = SingleSlater(mol, rec["basis"], ref="UHF", tau=1e-14, guess="SAD").SCF3()
         assert abs(r.energy - rec["E"]) < d["tolerance"], (test, r.energy, rec["E"])
+
+
+def test_incremental_fock_reaches_fp64_energy():
+    """Difference-density Fock build (G[P_k] = G[P_k-1] + G[P_k - P_k-1], full rebuild every
+    4 iterations) converges to the plain build's energy within 1e-9 Eh ((H2O)3 / cc-pVDZ)."""
+    from chronusq_public_amd.singleslater import SingleSlater
+    mol = M.water_cluster(3, seed=2, box_ang=M.scaled_box(3))
+    plain = SingleSlater(mol, "cc-pvdz", tau=1e-12).SCF3()
+    inc = SingleSlater(mol, "cc-pvdz", tau=1e-12, incremental=True, rebuild_every=4).SCF3()
+    assert plain.converged and inc.converged
+    assert abs(inc.energy - plain.energy) < 1e-9
