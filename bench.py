@@ -16,7 +16,7 @@ Extra fields:
   cpu_baseline  the CPU restatement of newTwoEContractDirect (oracle/, OpenMP over the
                 host cores) on the same G[D], timed once.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calibrate-copy]
 """
 from __future__ import annotations
 
@@ -70,21 +70,40 @@ def synthetic_density(bs, seed=0):
     return np.exp(-0.5 * R) * U
 
 
-def cpu_baseline(bs, D, tau, nthreads):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(bs, D, tau, nthreads, ncalls=3):
+    """CPU restatement (not Libint) of newTwoEContractDirect on the same G[D]: one warm-up
+    call, then the median of `ncalls` timed calls."""
     import oracle   # the CPU restatement, used only as the baseline leg
     ob = oracle.OracleBasis(bs)
     Q = ob.schwarz(nthreads)
     N = bs.nbasis
-    J = np.zeros((N, N), order="F")
-    K = np.zeros((N, N), order="F")
     Df = np.asfortranarray(D)
-    t = time.perf_counter()
-    n = ob.contract_direct(Q, tau, [Df, Df], [J, K], [0, 1], [1.0, -0.5], nthreads)
-    dt = time.perf_counter() - t
+    times = []
+    n = 0
+    for i in range(ncalls + 1):
+        J = np.zeros((N, N), order="F")
+        K = np.zeros((N, N), order="F")
+        t = time.perf_counter()
+        n = ob.contract_direct(Q, tau, [Df, Df], [J, K], [0, 1], [1.0, -0.5], nthreads)
+        if i:
+            times.append(time.perf_counter() - t)
+    dt = float(np.median(times))
     return {"value": n / dt, "unit": "quartets/s", "cores": nthreads, "kind": "port",
-            "sample": f"one full G[D] of the same workload ({n} surviving quartets) by the CPU "
-                      f"restatement of newTwoEContractDirect (Obara-Saika/HGP ERIs, OpenMP), "
-                      f"{dt:.2f} s"}
+            "cpu": f"{cpu_model()} (nproc {os.cpu_count()})",
+            "sample": f"CPU restatement (not Libint) of newTwoEContractDirect, Obara-Saika/HGP "
+                      f"ERIs, OpenMP: the same G[D] ({n} surviving quartets), 1 warm-up + median "
+                      f"of {ncalls} calls, {dt:.2f} s per call"}
 
 
 def main():
@@ -95,6 +114,9 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-12)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run three 256 MiB device-to-device copies (a plain copy kernel of "
+                         "known size, the scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path "
@@ -144,6 +166,14 @@ def main():
         torch.cuda.synchronize()
         scf[name]["time_to_1e-6_s"] = r.setup_s + time_to_energy(r, E_ref, 1e-6)
         scf[name]["wall_s"] = min(scf[name]["wall_s"], time.perf_counter() - t0)
+
+    if args.calibrate_copy:
+        src = torch.empty(32 << 20, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ao = AOIntegrals(bs)
     ao.thresholdSchwartz = args.tau
@@ -204,6 +234,8 @@ def main():
                           "quartets": int(list(counts.values())[k]), "prim_quartets": ws[4 * k],
                           "model_flop": fl, "kernel_ms": mean_ms,
                           "tflops": fl / (mean_ms * 1e-3) / 1e12 if mean_ms > 0 else 0.0})
+    prim_q = sum(ws[4 * k] for k in range(ncp))
+    n_int = sum(ws[4 * k + 3] for k in range(ncp))
     dom = max(per_class, key=lambda r: r["kernel_ms"])
     achieved = dom["model_flop"] / (dom["kernel_ms"] * 1e-3) / 1e12
     value = nquart / (ms_per_step * 1e-3)
@@ -215,6 +247,8 @@ def main():
         "config": {"workload": "RHF G[D] = 2J[P] - K[P], (H2O)16 seed 0 / cc-pVDZ", "nbf": N,
                    "nshell": bs.nshell, "tau": args.tau, "surviving_quartets": nquart,
                    "model_gflop_per_step": total_flops / 1e9,
+                   "prim_quartets_per_s": prim_q / (ms_per_step * 1e-3),
+                   "integrals_per_s": n_int / (ms_per_step * 1e-3),
                    "serial_kernels_ms": sum(r["kernel_ms"] for r in per_class),
                    "all_kernels_tflops": total_flops / (sum(r["kernel_ms"] for r in per_class) * 1e-3) / 1e12,
                    "step_tflops": total_flops / (ms_per_step * 1e-3) / 1e12},

@@ -43,8 +43,9 @@
 #include "_gen/eri_registry.inc"
 
 static ClassKernels kClassKernels[CQ_NUM_CLASS_PAIRS];
-// dynamic LDS of the segment kernels (static LDS: Boys table, 26 KB)
-static const size_t kMaxDynLds = 36 * 1024;
+// dynamic LDS of the segment kernels: what the 64 KiB per workgroup leaves beside the static
+// LDS (Boys table, 25992 B, and the work-item slot)
+static const size_t kMaxDynLds = 64 * 1024 - BOYS_NGRID * BOYS_LDS_W * sizeof(double) - 256;
 static const int kNumClassPairs = CQ_NUM_CLASS_PAIRS;
 static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 
@@ -52,15 +53,17 @@ static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 // Screening: one wavefront per (class pair, bra pair); compacted task lists.
 // ---------------------------------------------------------------------------------------
 struct CPMeta {
-  int bra_start, nbra, ket_start, nket, same, bra_class;
+  int bra_start, nbra, ket_start, nket, same, bra_class, ket_class;
+  int flip;   // 1: segments are keyed on the bra pair (bra-major task lists)
   long long task_off;
   long long seg_off;
 };
 
-// One wavefront per (class pair, ket pair); lanes sweep the bra pairs (sorted by Schwarz
-// bound, so the sweep stops once no bra can pass).  Survivors are staged in a per-wavefront
-// LDS buffer and flushed with one atomic per buffer, so task lists come out ket-major: the
-// lanes of an ERI wavefront then share one ket pair.
+// One wavefront per (class pair, uniform pair); the uniform pair is the ket (or the bra when
+// flip is set) and the lanes sweep the other class's pairs (sorted by Schwarz bound, so the
+// sweep stops once no pair can pass).  Survivors are staged in a per-wavefront LDS buffer and
+// flushed with one atomic per buffer, so task lists come out grouped by the uniform pair: the
+// lanes of an ERI wavefront then share it.  Tasks are stored as (bra, ket) either way.
 #define SCREEN_BUF 1024
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -82,7 +85,10 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   while (cp + 1 < ncp && item_start[cp + 1] <= wave) ++cp;
   const CPMeta m = cps[cp];
   const int ket_local = wave - item_start[cp];
-  const int K = m.ket_start + ket_local;
+  const int K = (m.flip ? m.bra_start : m.ket_start) + ket_local;   // the uniform pair
+  const int sw_start = m.flip ? m.ket_start : m.bra_start;           // the swept class
+  const int nsw = m.flip ? m.nket : m.nbra;
+  const int sw_class = m.flip ? m.ket_class : m.bra_class;
   const int i_begin = m.same ? ket_local : 0;
   const double Qk = pairQ[K];
   const int2 sk = s12[K];
@@ -103,17 +109,17 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     wave_sync_lds();
     nb = 0;
   };
-  const int g_lo = cls_grp_off[m.bra_class], g_hi = cls_grp_off[m.bra_class + 1];
+  const int g_lo = cls_grp_off[sw_class], g_hi = cls_grp_off[sw_class + 1];
   for (int g = g_lo; g < g_hi; ++g) {
-  const int grp_end = (g + 1 < g_hi) ? grp[g + 1] : m.nbra;
+  const int grp_end = (g + 1 < g_hi) ? grp[g + 1] : nsw;
   if (grp_end <= i_begin) continue;
   for (int i0 = max(grp[g], i_begin); i0 < grp_end; i0 += 64) {
-    const double Qi0 = pairQ[m.bra_start + i0];
+    const double Qi0 = pairQ[sw_start + i0];
     if (nglob * Qk * Qi0 * (1.0 + 1e-12) < tau) break;
     const int i = i0 + lane;
     bool pass = false;
     if (i < grp_end) {
-      const int P = m.bra_start + i;
+      const int P = sw_start + i;
       const double Qp = pairQ[P];
       const int2 sp = s12[P];
       const long canonP = (long)sp.x * (sp.x + 1) / 2 + sp.y;
@@ -130,7 +136,10 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     const unsigned long long mask = __ballot(pass);
     const int cnt = __popcll(mask);
     if (nb + cnt > SCREEN_BUF) flush();
-    if (pass) wbuf[nb + __popcll(mask & ((1ull << lane) - 1ull))] = make_uint2((unsigned)(m.bra_start + i), (unsigned)K);
+    if (pass) {
+      const unsigned P = (unsigned)(sw_start + i);
+      wbuf[nb + __popcll(mask & ((1ull << lane) - 1ull))] = m.flip ? make_uint2((unsigned)K, P) : make_uint2(P, (unsigned)K);
+    }
     nb += cnt;
   }
   }
@@ -475,9 +484,10 @@ struct cq_ctx {
   double tau = 1e-14;                // include/aointegrals.h:334
   int precision_mode = 0;
   double prim_threshold = 1e-17;
-  std::vector<CPMeta> cps;
-  std::vector<int> item_start;
-  int nitem = 0;
+  // class-pair tables: [0] every class pair ket-keyed, [1] with bra-keyed (flipped) class pairs
+  std::vector<CPMeta> cps[2];
+  std::vector<int> item_start[2];
+  int nitem[2] = {0, 0};
   long long task_capacity = 0;
   std::vector<unsigned long long> last_counts;
   // profiling: events around every class-pair kernel of every contraction
@@ -505,7 +515,7 @@ struct cq_ctx {
   DevBuf<double> d_alpha, d_coef, d_origin, d_onee;
   // device
   DevBuf<ShellMeta> d_shells;
-  DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start, d_flags;
+  DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start[2], d_flags;
   DevBuf<double> d_tco, d_boys, d_pairQ, d_Nmax, d_work, d_mats;
   DevBuf<PairInfo> d_pairs;
   DevBuf<int2> d_s12;
@@ -516,19 +526,19 @@ struct cq_ctx {
   hipStream_t aux[kAux] = {};
   hipEvent_t ev_fork = nullptr, ev_join[kAux] = {};
   DevBuf<PrimPair> d_prims;
-  DevBuf<CPMeta> d_cps;
+  DevBuf<CPMeta> d_cps[2];
   DevBuf<uint2> d_tasks;
   DevBuf<Segment> d_segs;
   DevBuf<unsigned> d_nseg, d_work_q;
-  bool use_lds = true;
+  int lds_mode = 2;   // 0 HBM atomics, 1 ket-keyed LDS segments, 2 also bra-keyed segments
   DevBuf<unsigned long long> d_counts, d_nglob;
   std::string err;
 
   ~cq_ctx() {
     d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
-    d_item_start.release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
+    d_item_start[0].release(); d_item_start[1].release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
     d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
-    d_cps.release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_nglob.release();
+    d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_nglob.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < kAux; ++k) {
@@ -751,11 +761,15 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     c->Q[(size_t)pi_.s1 * nshell + pi_.s2] = c->pairQ[k];
     c->Q[(size_t)pi_.s2 * nshell + pi_.s1] = c->pairQ[k];
   }
-  // class-pair metadata and task capacity
-  c->cps.clear();
-  c->item_start.clear();
-  long long off = 0, seg_total = 0;
-  int items = 0;
+  // class-pair metadata and task capacity; table 1 keys the segments of a class pair on the
+  // bra pair when the bra class has at most half as many pairs as the ket class (few bras
+  // survive per ket, many kets per bra) and the bra-keyed LDS accumulators fit
+  for (int t = 0; t < 2; ++t) {
+    c->cps[t].clear();
+    c->item_start[t].clear();
+  }
+  long long off = 0, seg_total[2] = {0, 0};
+  int items[2] = {0, 0};
   for (int k = 0; k < kNumClassPairs; ++k) {
     const ClassKernels& ck = kClassKernels[k];
     CPMeta m{};
@@ -765,22 +779,31 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     m.nket = c->class_start[ck.ki + 1] - c->class_start[ck.ki];
     m.same = ck.i == ck.j;
     m.bra_class = ck.bi;
+    m.ket_class = ck.ki;
     m.task_off = off;
     const long long cap = m.same ? (long long)m.nbra * (m.nbra + 1) / 2 : (long long)m.nbra * m.nket;
     off += cap;
-    // each ket's survivors are flushed in pieces of at most SCREEN_BUF tasks
-    m.seg_off = seg_total;
-    seg_total += m.nket + cap / SCREEN_BUF + 1;
-    c->cps.push_back(m);
-    c->item_start.push_back(items);
-    items += m.nket;
+    const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->ncart + (size_t)ck.na * ck.nb) * sizeof(double);
+    for (int t = 0; t < 2; ++t) {
+      CPMeta mt = m;
+      mt.flip = (t == 1 && !m.same && ck.jksegb && ldsb <= kMaxDynLds && 2LL * m.nbra <= m.nket) ? 1 : 0;
+      const int nuni = mt.flip ? m.nbra : m.nket;
+      // each uniform pair's survivors are flushed in pieces of at most SCREEN_BUF tasks
+      mt.seg_off = seg_total[t];
+      seg_total[t] += nuni + cap / SCREEN_BUF + 1;
+      c->cps[t].push_back(mt);
+      c->item_start[t].push_back(items[t]);
+      items[t] += nuni;
+    }
   }
-  c->nitem = items;
   c->task_capacity = off;
-  c->d_cps.upload(c->cps, c->stream);
-  c->d_item_start.upload(c->item_start, c->stream);
+  for (int t = 0; t < 2; ++t) {
+    c->nitem[t] = items[t];
+    c->d_cps[t].upload(c->cps[t], c->stream);
+    c->d_item_start[t].upload(c->item_start[t], c->stream);
+  }
   c->d_tasks.ensure((size_t)std::max<long long>(1, off));
-  c->d_segs.ensure((size_t)std::max<long long>(1, seg_total));
+  c->d_segs.ensure((size_t)std::max<long long>(1, std::max(seg_total[0], seg_total[1])));
   c->d_nseg.ensure(kNumClassPairs);
   c->d_work_q.ensure(kNumClassPairs);
   c->d_counts.ensure(kNumClassPairs);
@@ -860,9 +883,12 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   HIPCHK(hipMemsetAsync(c->d_counts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->d_nseg.p, 0, kNumClassPairs * sizeof(unsigned), s));
   HIPCHK(hipMemsetAsync(c->d_work_q.p, 0, kNumClassPairs * sizeof(unsigned), s));
-  if (c->nitem > 0) {
-    hipLaunchKernelGGL(screen_kernel, dim3(grid_for(c->nitem, WG / 64)), dim3(WG), 0, s, c->d_cps.p,
-                       kNumClassPairs, c->d_item_start.p, c->nitem, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
+  // bra-keyed segments need the segment kernels for every pass: one density slot, LDS on
+  const int tb = (c->lds_mode >= 2 && nslot == 1) ? 1 : 0;
+  const std::vector<CPMeta>& cps = c->cps[tb];
+  if (c->nitem[tb] > 0) {
+    hipLaunchKernelGGL(screen_kernel, dim3(grid_for(c->nitem[tb], WG / 64)), dim3(WG), 0, s, c->d_cps[tb].p,
+                       kNumClassPairs, c->d_item_start[tb].p, c->nitem[tb], c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
                        c->d_cls_grp_off.p, c->d_Nmax.p,
                        c->d_nglob.p, ns, c->tau, c->d_tasks.p, c->d_counts.p, c->d_segs.p, c->d_nseg.p);
     HIPCHK(hipGetLastError());
@@ -891,28 +917,36 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
     for (int kk = kNumClassPairs - 1; kk >= 0; --kk) {
       const int k = c->launch_order[kk];
       const ClassKernels& ck = kClassKernels[k];
-      if (c->cps[k].nbra == 0 || c->cps[k].nket == 0) continue;
+      const CPMeta& m = cps[k];
+      if (m.nbra == 0 || m.nket == 0) continue;
       hipStream_t sk = c->aux[launched++ % c->n_streams];
-      const uint2* tasks = c->d_tasks.p + c->cps[k].task_off;
-      const long long cap = c->cps[k].same ? (long long)c->cps[k].nbra * (c->cps[k].nbra + 1) / 2
-                                           : (long long)c->cps[k].nbra * c->cps[k].nket;
+      const uint2* tasks = c->d_tasks.p + m.task_off;
+      const long long cap = m.same ? (long long)m.nbra * (m.nbra + 1) / 2 : (long long)m.nbra * m.nket;
       const long long items = ((cap + 63) / 64) * 64 * ck.nchunk;
       const int grid = (int)std::min<long long>(persistent_grid(), (items + WG - 1) / WG);
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
       // one density slot and LDS room for the K columns: segment kernel (LDS accumulation)
       const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->ncart + (size_t)ck.nc * ck.nd) * sizeof(double);
-      const bool seg_path = c->use_lds && ng == 1 && lds_bytes <= kMaxDynLds;
+      const bool seg_path = c->lds_mode >= 1 && ng == 1 && lds_bytes <= kMaxDynLds;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (c->profiling) {
         e0 = c->get_event();
         e1 = c->get_event();
         HIPCHK(hipEventRecord(e0, sk));
       }
-      if (seg_path) {
-        const long long seg_cap = c->cps[k].nket + cap / SCREEN_BUF + 1;
+      if (m.flip) {
+        // bra-keyed segments (table 1 only flips where the LDS rows fit)
+        const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->ncart + (size_t)ck.na * ck.nb) * sizeof(double);
+        const long long seg_cap = m.nbra + cap / SCREEN_BUF + 1;
+        const int sgrid = (int)std::min<long long>(persistent_grid(), seg_cap * ck.nchunk);
+        hipLaunchKernelGGL(ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
+                           c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
+                           c->d_prims.p, c->d_boys.p, jk);
+      } else if (seg_path) {
+        const long long seg_cap = m.nket + cap / SCREEN_BUF + 1;
         const int sgrid = (int)std::min<long long>(persistent_grid(), seg_cap * ck.nchunk);
         hipLaunchKernelGGL(ck.jkseg, dim3(std::max(1, sgrid)), dim3(WG), lds_bytes, sk, tasks,
-                           c->d_segs.p + c->cps[k].seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
+                           c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
       } else {
         hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
@@ -1146,8 +1180,11 @@ int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, 
 
 // Selects the J/K digestion: 1 (default) accumulates K columns and J(3,4) of each ket pair in
 // LDS and flushes once per work item; 0 issues every update as an HBM atomic.
-int cq_set_lds_digestion(cq_ctx_t* c, int on) {
-  return guarded(c, [&] { c->use_lds = on != 0; });
+int cq_set_lds_digestion(cq_ctx_t* c, int mode) {
+  return guarded(c, [&] {
+    if (mode < 0 || mode > 2) throw std::invalid_argument("digestion mode must be 0, 1 or 2");
+    c->lds_mode = mode;
+  });
 }
 
 // Number of streams the class-pair kernels are spread over (1 serialises them, for
@@ -1200,9 +1237,9 @@ int cq_get_work_stats(cq_ctx_t* c, double* out, int max_n) {
     acc.ensure((size_t)4 * kNumClassPairs);
     HIPCHK(hipMemsetAsync(acc.p, 0, 4 * kNumClassPairs * sizeof(double), c->stream));
     for (int k = 0; k < kNumClassPairs; ++k) {
-      if (c->cps[k].nbra == 0 || c->cps[k].nket == 0) continue;
+      if (c->cps[0][k].nbra == 0 || c->cps[0][k].nket == 0) continue;
       hipLaunchKernelGGL(work_stats_kernel, dim3(persistent_grid()), dim3(WG), 0, c->stream,
-                         c->d_tasks.p + c->cps[k].task_off, c->d_counts.p + k, c->d_pairs.p, c->d_shells.p,
+                         c->d_tasks.p + c->cps[0][k].task_off, c->d_counts.p + k, c->d_pairs.p, c->d_shells.p,
                          acc.p + 4 * k);
       HIPCHK(hipGetLastError());
     }

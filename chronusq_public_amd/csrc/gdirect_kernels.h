@@ -48,6 +48,25 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
   q.ket_swap = K.a != K.s1;
 }
 
+// Quartet of the wavefront-uniform bra pair bb (from readfirstlane) with one lane's ket pair K;
+// lanes that do not own a task get an empty ket range and degeneracy 0.
+__device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsigned ket_idx, unsigned bb, bool mine,
+                                                         const PairInfo* __restrict__ pairs,
+                                                         const PrimPair* __restrict__ prims, QuartetArgs& q) {
+  const PairInfo B = pairs[bb];
+  q.prim = prims;
+  q.bra_p0 = B.p0;
+  q.bra_p1 = B.p1;
+  q.ket_p0 = K.p0;
+  q.ket_p1 = mine ? K.p1 : K.p0;
+  q.a0 = B.a0; q.b0 = B.b0; q.c0 = K.a0; q.d0 = K.b0;
+  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (bb == ket_idx ? 1.0 : 2.0);
+  q.deg = mine ? deg : 0.0;
+  q.ket_uniform = false;
+  q.bra_swap = B.a != B.s1;
+  q.ket_swap = K.a != K.s1;
+}
+
 // JK kernel: work items are (chunk, 64-task block) pairs, one chunk per wavefront, so classes
 // whose ket components are split into independent chunks run them in parallel.
 #define DEFINE_CLASS_KERNELS(I, J, BI, KI, NAME, LTOT)                                          \
@@ -82,7 +101,7 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
         const bool mine = pending && tk.y == kk;                                                \
         QuartetArgs q;                                                                          \
         make_quartet_uniform_ket(B, tk.x, kk, mine, pairs, prims, q);                           \
-        quartet_jk_##NAME<NSLOT, false>(q, jk, boys_lds, chunk, nullptr, nullptr);                                       \
+        quartet_jk_##NAME<NSLOT, 0>(q, jk, boys_lds, chunk, nullptr, nullptr);                  \
         pending = pending && !mine;                                                             \
       }                                                                                         \
     }                                                                                           \
@@ -124,7 +143,7 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
         const PairInfo B = pairs[tk.x];                                                       \
         QuartetArgs q;                                                                        \
         make_quartet_uniform_ket(B, tk.x, kk, active, pairs, prims, q);                       \
-        quartet_jk_##NAME<1, true>(q, jk, boys_lds, chunk, colK, jcdL);                       \
+        quartet_jk_##NAME<1, 1>(q, jk, boys_lds, chunk, colK, jcdL);                          \
       }                                                                                       \
       __syncthreads();                                                                        \
       const PairInfo K = pairs[kk];                                                           \
@@ -166,6 +185,71 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
     }                                                                                           \
   }
 
+// Bra-uniform segment kernel (one density slot), for class pairs whose bra class has few
+// pairs: a segment is one bra pair with the ket pairs that survive against it.  K rows a/b
+// (NA + NB rows of ncart) and J(1,2) accumulate in LDS and are flushed once per item; J(3,4)
+// goes to HBM per lane.
+#define DEFINE_CLASS_SEGB(NAME, LTOT)                                                           \
+  __global__ void __launch_bounds__(WG) eri_jksegb_##NAME(                                      \
+      const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                        \
+      const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                       \
+      const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                   \
+      const double* __restrict__ boys, JKArgs jk) {                                             \
+    constexpr int NCH = kChunks_##NAME, NA = kNA_##NAME, NB = kNB_##NAME, NROW = NA + NB;       \
+    __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                        \
+    __shared__ int s_item;                                                                      \
+    extern __shared__ double dyn_lds[];                                                         \
+    const long nitems = (long)(*nseg_ptr) * NCH;                                                \
+    if (nitems == 0) return;                                                                    \
+    stage_boys(boys, LTOT, boys_lds);                                                           \
+    const int N = jk.ncart;                                                                     \
+    double* rowK = dyn_lds;                                                                     \
+    double* jabL = dyn_lds + NROW * N;                                                          \
+    for (int k = threadIdx.x; k < NROW * N + NA * NB; k += WG) dyn_lds[k] = 0.0;               \
+    const int wofs = threadIdx.x & ~63;                                                         \
+    while (true) {                                                                              \
+      __syncthreads();                                                                          \
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(work, 1u);                                  \
+      __syncthreads();                                                                          \
+      const long item = s_item;                                                                 \
+      if (item >= nitems) break;                                                                \
+      const Segment sg = segs[item / NCH];                                                      \
+      const int chunk = (int)(item % NCH);                                                      \
+      const unsigned bb = __builtin_amdgcn_readfirstlane((unsigned)sg.ket);                     \
+      for (int t0 = 0; t0 < sg.count; t0 += WG) {                                               \
+        if (t0 + wofs >= sg.count) continue;                                                    \
+        const int t = t0 + threadIdx.x;                                                         \
+        const bool active = t < sg.count;                                                       \
+        const uint2 tk = tasks[sg.off + (active ? t : t0 + wofs)];                              \
+        const PairInfo K = pairs[tk.y];                                                         \
+        QuartetArgs q;                                                                          \
+        make_quartet_uniform_bra(K, tk.y, bb, active, pairs, prims, q);                         \
+        quartet_jk_##NAME<1, 2>(q, jk, boys_lds, chunk, rowK, jabL);                            \
+      }                                                                                         \
+      __syncthreads();                                                                          \
+      const PairInfo A = pairs[bb];                                                             \
+      for (int k = threadIdx.x; k < NROW * N; k += WG) {                                        \
+        const double v = rowK[k];                                                               \
+        if (v != 0.0) {                                                                         \
+          const int lrow = k / N, col = k - lrow * N;                                           \
+          const int grow = lrow < NA ? A.a0 + lrow : A.b0 + (lrow - NA);                        \
+          atomicAdd(&jk.GK[0][(long)grow * N + col], v);                                        \
+          rowK[k] = 0.0;                                                                        \
+        }                                                                                       \
+      }                                                                                         \
+      if (threadIdx.x < NA * NB) {                                                              \
+        const double v = jabL[threadIdx.x];                                                     \
+        if (v != 0.0) {                                                                         \
+          const bool bs = A.a != A.s1;                                                          \
+          const int ia = threadIdx.x / NB, ib = threadIdx.x - (threadIdx.x / NB) * NB;          \
+          const int tab = bs ? A.b0 * N + A.a0 : A.a0 * N + A.b0;                               \
+          atomicAdd(&jk.GJ[0][tab + ia * (bs ? 1 : N) + ib * (bs ? N : 1)], v);                 \
+          jabL[threadIdx.x] = 0.0;                                                              \
+        }                                                                                       \
+      }                                                                                         \
+    }                                                                                           \
+  }
+
 typedef void (*JKKernel)(const uint2*, const unsigned long long*, const PairInfo*, const PrimPair*,
                          const double*, JKArgs);
 typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int);
@@ -178,7 +262,8 @@ struct ClassKernels {
   JKKernel jk1, jk2;
   BlockKernel block;
   SegKernel jkseg;
-  int nc, nd;
+  SegKernel jksegb;   // bra-uniform segment kernel, or null
+  int nc, nd, na, nb;
 };
 
 

@@ -205,7 +205,11 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
     em.ind = 0
     em(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}) kind={kind} chunks={len(chunks)} persist={persist}")
     if kind == "jk":
-        em("template <int NSLOT, bool LDSK>")
+        # DIG: 0 = every J/K update is an HBM atomic; 1 = the ket is wavefront-uniform: K columns
+        # (c, d) accumulate in LDS (colK) and J(3,4) is reduced across the wavefront into LDS (jcdL);
+        # 2 = the bra is wavefront-uniform: K rows (a, b) accumulate in LDS (colK holds the rows)
+        # and J(1,2) is reduced into LDS (jcdL holds J(a,b)), J(3,4) is a per-lane HBM atomic.
+        em("template <int NSLOT, int DIG>")
         em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds, int chunk, double* colK, double* jcdL) {{")
     else:
         em(f"__device__ __forceinline__ void quartet_block_{name}(const QuartetArgs& q, double* __restrict__ out, const double* __restrict__ boys_lds) {{")
@@ -328,7 +332,7 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
     em("}")
     if kind == "jk":
         em(f"constexpr int kChunks_{name} = {1 if persist else len(chunks)};")
-        em(f"constexpr int kNC_{name} = {NC}, kND_{name} = {ND};")
+        em(f"constexpr int kNC_{name} = {NC}, kND_{name} = {ND}, kNA_{name} = {NA}, kNB_{name} = {NB};")
     return "\n".join(em.lines) + "\n"
 
 
@@ -395,12 +399,14 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
                 em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('xba', ia, ib)}];")
                 tgt = f"jab_{ia}_{ib}[s]" if persist else f"jab_{ia}_{ib}"
                 em(f"  {tgt} += o_{ia}_{ib}_{ic}_{idd} * x43;")
-        em(f"  if constexpr (LDSK) wave_add_lds(&jcdL[s * {NC * ND} + {ic * ND + idd}], dg * jcd);")
+        em(f"  if constexpr (DIG == 1) wave_add_lds(&jcdL[s * {NC * ND} + {ic * ND + idd}], dg * jcd);")
+        em(f"  else if constexpr (DIG == 2) {{ if (live) atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
         em(f"  else wave_add(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd, q.ket_uniform); }}")
     if not persist:
         for ia in range(NA):
             for ib in range(NB):
-                em(f"if (live) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib});")
+                em(f"if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], dg * jab_{ia}_{ib});")
+                em(f"else if (live) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib});")
     em("if (ns) {")
     em.ind += 1
     for (ic, idd) in ks:
@@ -435,7 +441,10 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
             return f"if (live) {glob};"
         lcol = ci if colkind == "c" else NC + ci
         lds = f"atomicAdd(&colK[(s * {NCOL} + {lcol}) * N + q.{row0} + {ri}], {val})"
-        return f"if (live) {{ if constexpr (LDSK) {lds}; else {glob}; }}"
+        lrow = ri if row0 == "a0" else NA + ri
+        ldsb = f"atomicAdd(&colK[(s * {NA + NB} + {lrow}) * N + {col0} + {ci}], {val})"
+        return (f"if (live) {{ if constexpr (DIG == 1) {lds}; else if constexpr (DIG == 2) {ldsb}; "
+                f"else {glob}; }}")
 
     def kupdates(G, trans, with_ad_bd):
         # K(a,c) += X(b,d) ; K(b,c) += X(a,d)   (partial over the chunk's d)
@@ -513,19 +522,22 @@ def gen_digest_final(em: Emitter, NA, NB, NC, ND):
     em("const int tab_a = q.bra_swap ? 1 : N, tab_b = q.bra_swap ? N : 1;")
     em("const double dg = q.deg;")
     em("const double hk = 0.5 * q.deg;")
-    em("if (q.deg != 0.0 && jk.do_j[s]) {")
+    em("if (jk.do_j[s]) {")
     for ia in range(NA):
         for ib in range(NB):
-            em(f"  atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}[s]);")
+            em(f"  if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], dg * jab_{ia}_{ib}[s]);")
+            em(f"  else if (q.deg != 0.0) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}[s]);")
     em("}")
     em("if (q.deg != 0.0 && jk.do_k[s]) {")
     for ia in range(NA):
         for idd in range(ND):
-            em(f"  if constexpr (LDSK) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.a0 + {ia}], hk * kad_{ia}_{idd}[s]);")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.a0 + {ia}], hk * kad_{ia}_{idd}[s]);")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
             em(f"  else atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
     for ib in range(NB):
         for idd in range(ND):
-            em(f"  if constexpr (LDSK) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.b0 + {ib}], hk * kbd_{ib}_{idd}[s]);")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.b0 + {ib}], hk * kbd_{ib}_{idd}[s]);")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {NA + ib}) * N + q.d0 + {idd}], hk * kbd_{ib}_{idd}[s]);")
             em(f"  else atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], hk * kbd_{ib}_{idd}[s]);")
     em("}")
     em.ind -= 1
@@ -569,9 +581,15 @@ def main(out_dir: str, maxl: int = 2):
             src.append(gen_class(LA, LB, LC, LD, "jk"))
             src.append(gen_class(LA, LB, LC, LD, "block"))
             src.append(f"DEFINE_CLASS_KERNELS({i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {name}, {L})")
+            # bra-uniform segments pay off where the bra class has far fewer pairs than the ket
+            # class (few bras survive per ket): different pair classes and a bra above (ss)
+            segb = i != j and (LA, LB) != (0, 0)
+            if segb:
+                src.append(f"DEFINE_CLASS_SEGB({name}, {L})")
             src.append(f"void cq_register_{name}(ClassKernels& ck) {{")
             src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, kChunks_{name}, \"{name}\", "
-                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}, kNC_{name}, kND_{name}}};")
+                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}, "
+                       f"{('eri_jksegb_' + name) if segb else 'nullptr'}, kNC_{name}, kND_{name}, kNA_{name}, kNB_{name}}};")
             src.append("}")
             _write_if_changed(os.path.join(out_dir, f"eri_{name}.hip"), "\n".join(src) + "\n")
             table.append(name)

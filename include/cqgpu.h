@@ -94,9 +94,11 @@ int cq_get_quartet_counts(cq_ctx_t* ctx, long long* counts, int max_counts, int*
 int cq_num_class_pairs(void);
 int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld);
 
-/* J/K digestion mode: 1 (default) accumulates each ket pair's K columns and J block in LDS and
- * flushes them once per work item; 0 sends every update to HBM as an FP64 atomic. */
-int cq_set_lds_digestion(cq_ctx_t* ctx, int on);
+/* J/K digestion mode: 2 (default) groups the tasks of each class pair by the pair class with
+ * fewer pairs (bra or ket) and accumulates that pair's K rows/columns and J block in LDS,
+ * flushed once per work item; 1 always groups by the ket pair; 0 sends every update to HBM as
+ * an FP64 atomic.  Results agree to rounding in every mode. */
+int cq_set_lds_digestion(cq_ctx_t* ctx, int mode);
 
 /* Number of HIP streams (1..4, default 4) the class-pair kernels of one contraction are spread
  * over; 1 runs them one after another (for per-kernel timing). */

@@ -196,11 +196,12 @@ def test_full_size_water16_properties():
         assert np.abs(g - ob.quartet(*q)).max() < 1e-12
 
 
-@pytest.mark.parametrize("lds", [0, 1])
-def test_both_digestion_modes_match_oracle(lds):
-    """HBM-atomic and LDS-accumulated digestion against the oracle ((H2O)3 / cc-pVDZ)."""
+@pytest.mark.parametrize("lds", [0, 1, 2])
+def test_every_digestion_mode_matches_oracle(lds):
+    """HBM-atomic, ket-keyed and bra-keyed LDS digestion against the oracle ((H2O)8 / cc-pVDZ:
+    bra-keyed (dd|ss) segments hold more than one screening buffer of kets)."""
     from chronusq_public_amd import _lib
-    mol = M.water_cluster(3, seed=11, box_ang=M.scaled_box(3))
+    mol = M.water_cluster(8, seed=11, box_ang=M.scaled_box(8))
     bs, ao, ob = _setup(mol, "cc-pvdz")
     _lib.check(_lib.load().cq_set_lds_digestion(ao._h, lds), ao._h)
     X = _sym_random(bs.nbasis, 5)
