@@ -56,7 +56,9 @@ struct CPMeta {
   int bra_start, nbra, ket_start, nket, same, bra_class, ket_class;
   int flip;   // 1: segments are keyed on the bra pair (bra-major task lists)
   long long task_off;
+  long long task_cap;   // task entries (primitive blocks of the swept pairs included)
   long long seg_off;
+  long long seg_cap;
 };
 
 // One wavefront per (class pair, uniform pair); the uniform pair is the ket (or the bra when
@@ -74,8 +76,10 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     const CPMeta* __restrict__ cps, int ncp, const int* __restrict__ item_start, int nitem,
     const int2* __restrict__ s12, const double* __restrict__ pairQ, const int* __restrict__ grp,
     const int* __restrict__ cls_grp_off, const double* __restrict__ Nmax,
-    const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, uint2* __restrict__ tasks,
-    unsigned long long* __restrict__ counts, Segment* __restrict__ segs, unsigned* __restrict__ nseg) {
+    const unsigned long long* __restrict__ nglob_bits, int nshell, double tau,
+    const unsigned char* __restrict__ pair_nblk, uint2* __restrict__ tasks,
+    unsigned long long* __restrict__ counts, unsigned long long* __restrict__ qcounts,
+    Segment* __restrict__ segs, unsigned* __restrict__ nseg) {
   __shared__ uint2 buf[WG / 64][SCREEN_BUF];
   const int wid = threadIdx.x >> 6;
   const int wave = blockIdx.x * (WG / 64) + wid;
@@ -96,15 +100,20 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   const double nglob = __longlong_as_double((long long)*nglob_bits);
   uint2* wbuf = buf[wid];
   int nb = 0;
+  unsigned long long nq = 0;
+  // a uniform pair with more than PRIM_BLOCK primitive pairs gets one segment per block
+  const int ublk = pair_nblk[K];
   auto flush = [&]() {
     wave_sync_lds();
     unsigned long long base = 0;
+    unsigned si = 0;
     if (lane == 0) {
       base = atomicAdd(&counts[cp], (unsigned long long)nb);
-      const unsigned si = atomicAdd(&nseg[cp], 1u);
-      segs[m.seg_off + si] = Segment{(int)base, nb, K, 0};
+      si = atomicAdd(&nseg[cp], (unsigned)ublk);
     }
     base = __shfl(base, 0);
+    si = __shfl(si, 0);
+    if (lane < ublk) segs[m.seg_off + si + lane] = Segment{(int)base, nb, K, ublk > 1 ? lane + 1 : 0};
     for (int k = lane; k < nb; k += 64) tasks[m.task_off + base + k] = wbuf[k];
     wave_sync_lds();
     nb = 0;
@@ -133,17 +142,25 @@ __global__ void __launch_bounds__(WG) screen_kernel(
       const double mx = fmax(n14, fmax(n24, fmax(n34, fmax(n13, fmax(n23, n12)))));
       pass = !(__dmul_rn(__dmul_rn(mx, Q12), Q34) < tau);
     }
-    const unsigned long long mask = __ballot(pass);
-    const int cnt = __popcll(mask);
-    if (nb + cnt > SCREEN_BUF) flush();
-    if (pass) {
-      const unsigned P = (unsigned)(sw_start + i);
-      wbuf[nb + __popcll(mask & ((1ull << lane) - 1ull))] = m.flip ? make_uint2((unsigned)K, P) : make_uint2(P, (unsigned)K);
+    nq += __popcll(__ballot(pass));
+    // one task entry per primitive block of the swept pair
+    const int pblk = pass ? pair_nblk[sw_start + i] : 0;
+    for (int j = 0;; ++j) {
+      const bool e = j < pblk;
+      const unsigned long long mask = __ballot(e);
+      if (mask == 0ull) break;
+      const int cnt = __popcll(mask);
+      if (nb + cnt > SCREEN_BUF) flush();
+      if (e) {
+        const unsigned P = (unsigned)(sw_start + i) | (pblk > 1 ? (unsigned)(j + 1) << 24 : 0u);
+        wbuf[nb + __popcll(mask & ((1ull << lane) - 1ull))] = m.flip ? make_uint2((unsigned)K, P) : make_uint2(P, (unsigned)K);
+      }
+      nb += cnt;
     }
-    nb += cnt;
   }
   }
   if (nb > 0) flush();
+  if (lane == 0 && nq > 0) atomicAdd(&qcounts[cp], nq);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -324,10 +341,16 @@ __global__ void work_stats_kernel(const uint2* __restrict__ tasks, const unsigne
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
     const uint2 tk = tasks[t];
-    const PairInfo B = pairs[tk.x], K = pairs[tk.y];
-    const double kab = B.p1 - B.p0, kcd = K.p1 - K.p0;
+    const PairInfo B = pairs[pair_of(tk.x)], K = pairs[pair_of(tk.y)];
+    int bp0, bp1, kp0, kp1;
+    prim_range(B, block_of(tk.x), bp0, bp1);
+    prim_range(K, block_of(tk.y), kp0, kp1);
+    const double kab = bp1 - bp0, kcd = kp1 - kp0;
     s0 += kab * kcd;
     s1 += kab;
+    // per-quartet terms once per quartet: entries of a split pair other than its first block
+    // are further primitive blocks of the same quartet
+    if (block_of(tk.x) > 1 || block_of(tk.y) > 1) continue;
     const int shl[4] = {B.a, B.b, K.a, K.b};
     double dims[4];
     int l4[4], p4[4];
@@ -531,14 +554,15 @@ struct cq_ctx {
   DevBuf<Segment> d_segs;
   DevBuf<unsigned> d_nseg, d_work_q;
   int lds_mode = 2;   // 0 HBM atomics, 1 ket-keyed LDS segments, 2 also bra-keyed segments
-  DevBuf<unsigned long long> d_counts, d_nglob;
+  DevBuf<unsigned long long> d_counts, d_qcounts, d_nglob;
+  DevBuf<unsigned char> d_pair_nblk;
   std::string err;
 
   ~cq_ctx() {
     d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
     d_item_start[0].release(); d_item_start[1].release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
     d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
-    d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_nglob.release();
+    d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_qcounts.release(); d_nglob.release(); d_pair_nblk.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < kAux; ++k) {
@@ -768,6 +792,23 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     c->cps[t].clear();
     c->item_start[t].clear();
   }
+  // primitive blocks per pair (1: unsplit)
+  std::vector<unsigned char> nblk(npair);
+  for (int k = 0; k < npair; ++k) {
+    const int np = c->pairs[k].p1 - c->pairs[k].p0;
+    if (np > 64 * PRIM_BLOCK) throw std::invalid_argument("more than 512 primitive pairs in one shell pair");
+    nblk[k] = (unsigned char)(np > PRIM_BLOCK ? (np + PRIM_BLOCK - 1) / PRIM_BLOCK : 1);
+  }
+  if ((unsigned)npair > PAIR_MASK) throw std::invalid_argument("too many shell pairs");
+  c->d_pair_nblk.upload(nblk, c->stream);
+  auto class_blocks = [&](int cls, long long& sum, int& mx) {
+    sum = 0;
+    mx = 1;
+    for (int k = c->class_start[cls]; k < c->class_start[cls + 1]; ++k) {
+      sum += nblk[k];
+      mx = std::max<int>(mx, nblk[k]);
+    }
+  };
   long long off = 0, seg_total[2] = {0, 0};
   int items[2] = {0, 0};
   for (int k = 0; k < kNumClassPairs; ++k) {
@@ -780,17 +821,33 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     m.same = ck.i == ck.j;
     m.bra_class = ck.bi;
     m.ket_class = ck.ki;
-    m.task_off = off;
-    const long long cap = m.same ? (long long)m.nbra * (m.nbra + 1) / 2 : (long long)m.nbra * m.nket;
-    off += cap;
+    long long bsum, ksum;
+    int bmax, kmax;
+    class_blocks(ck.bi, bsum, bmax);
+    class_blocks(ck.ki, ksum, kmax);
+    // task entries: every swept pair's blocks against every uniform pair (same class: bra >= ket)
+    long long cap_ket = 0;
+    if (m.same) {
+      for (int i = 0; i < m.nbra; ++i) cap_ket += (long long)nblk[m.bra_start + i] * (i + 1);
+    } else {
+      cap_ket = bsum * m.nket;
+    }
     const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->ncart + (size_t)ck.na * ck.nb) * sizeof(double);
+    const int flip = (!m.same && ck.jksegb && ldsb <= kMaxDynLds && 2LL * m.nbra <= m.nket) ? 1 : 0;
+    const long long cap_bra = flip ? ksum * m.nbra : 0;
+    m.task_off = off;
+    off += std::max(cap_ket, cap_bra);
     for (int t = 0; t < 2; ++t) {
       CPMeta mt = m;
-      mt.flip = (t == 1 && !m.same && ck.jksegb && ldsb <= kMaxDynLds && 2LL * m.nbra <= m.nket) ? 1 : 0;
+      mt.flip = t == 1 ? flip : 0;
       const int nuni = mt.flip ? m.nbra : m.nket;
-      // each uniform pair's survivors are flushed in pieces of at most SCREEN_BUF tasks
+      mt.task_cap = mt.flip ? cap_bra : cap_ket;
+      // segments: a uniform pair with T tasks is flushed at most T / (SCREEN_BUF - 64) + 1 times
+      // (a flush happens once fewer than 64 slots are left), each flush writing one segment per
+      // primitive block of the uniform pair
+      mt.seg_cap = (mt.flip ? bsum : ksum) + (mt.task_cap / (SCREEN_BUF - 64) + 1) * (mt.flip ? bmax : kmax);
       mt.seg_off = seg_total[t];
-      seg_total[t] += nuni + cap / SCREEN_BUF + 1;
+      seg_total[t] += mt.seg_cap;
       c->cps[t].push_back(mt);
       c->item_start[t].push_back(items[t]);
       items[t] += nuni;
@@ -807,6 +864,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_nseg.ensure(kNumClassPairs);
   c->d_work_q.ensure(kNumClassPairs);
   c->d_counts.ensure(kNumClassPairs);
+  c->d_qcounts.ensure(kNumClassPairs);
   c->d_nglob.ensure(1);
   c->d_Nmax.ensure((size_t)nshell * nshell);
   c->last_counts.assign(kNumClassPairs, 0);
@@ -881,6 +939,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   }
   // 3. screening into compacted task lists
   HIPCHK(hipMemsetAsync(c->d_counts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->d_qcounts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->d_nseg.p, 0, kNumClassPairs * sizeof(unsigned), s));
   HIPCHK(hipMemsetAsync(c->d_work_q.p, 0, kNumClassPairs * sizeof(unsigned), s));
   // bra-keyed segments need the segment kernels for every pass: one density slot, LDS on
@@ -890,7 +949,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
     hipLaunchKernelGGL(screen_kernel, dim3(grid_for(c->nitem[tb], WG / 64)), dim3(WG), 0, s, c->d_cps[tb].p,
                        kNumClassPairs, c->d_item_start[tb].p, c->nitem[tb], c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
                        c->d_cls_grp_off.p, c->d_Nmax.p,
-                       c->d_nglob.p, ns, c->tau, c->d_tasks.p, c->d_counts.p, c->d_segs.p, c->d_nseg.p);
+                       c->d_nglob.p, ns, c->tau, c->d_pair_nblk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p,
+                       c->d_segs.p, c->d_nseg.p);
     HIPCHK(hipGetLastError());
   }
   // 4. ERI + J/K digestion, class pair by class pair, two density slots per pass
@@ -921,7 +981,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       if (m.nbra == 0 || m.nket == 0) continue;
       hipStream_t sk = c->aux[launched++ % c->n_streams];
       const uint2* tasks = c->d_tasks.p + m.task_off;
-      const long long cap = m.same ? (long long)m.nbra * (m.nbra + 1) / 2 : (long long)m.nbra * m.nket;
+      const long long cap = m.task_cap;
       const long long items = ((cap + 63) / 64) * 64 * ck.nchunk;
       const int grid = (int)std::min<long long>(persistent_grid(), (items + WG - 1) / WG);
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
@@ -937,14 +997,12 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       if (m.flip) {
         // bra-keyed segments (table 1 only flips where the LDS rows fit)
         const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->ncart + (size_t)ck.na * ck.nb) * sizeof(double);
-        const long long seg_cap = m.nbra + cap / SCREEN_BUF + 1;
-        const int sgrid = (int)std::min<long long>(persistent_grid(), seg_cap * ck.nchunk);
+        const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
         hipLaunchKernelGGL(ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
                            c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
       } else if (seg_path) {
-        const long long seg_cap = m.nket + cap / SCREEN_BUF + 1;
-        const int sgrid = (int)std::min<long long>(persistent_grid(), seg_cap * ck.nchunk);
+        const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
         hipLaunchKernelGGL(ck.jkseg, dim3(std::max(1, sgrid)), dim3(WG), lds_bytes, sk, tasks,
                            c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
@@ -1119,7 +1177,7 @@ int cq_get_quartet_counts(cq_ctx_t* c, long long* counts, int max_counts, int* n
   return guarded(c, [&] {
     HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<unsigned long long> h(kNumClassPairs);
-    HIPCHK(hipMemcpy(h.data(), c->d_counts.p, kNumClassPairs * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h.data(), c->d_qcounts.p, kNumClassPairs * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (int k = 0; k < kNumClassPairs && k < max_counts; ++k) counts[k] = (long long)h[k];
     if (n_out) *n_out = kNumClassPairs;
   });

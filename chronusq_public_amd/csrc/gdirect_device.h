@@ -125,8 +125,27 @@ __device__ __forceinline__ void stage_boys(const double* __restrict__ tab, int L
 // Work unit of the segment kernels: `count` tasks of one ket pair starting at `off` in the
 // class pair's task list.
 struct Segment {
-  int off, count, ket, pad;
+  int off, count, ket, blk;   // ket: the uniform pair; blk: its primitive-block code
 };
+
+// Primitive blocks.  A pair with more than PRIM_BLOCK primitive pairs is split into blocks of
+// PRIM_BLOCK so that one heavily contracted quartet becomes several shorter lane tasks (the
+// digestion is linear in the integrals, so partial sums digest independently).  Task entries
+// and segments carry a block code in bits 24..31 of the pair index: 0 = the whole primitive
+// range, k >= 1 = primitives [p0 + (k-1) PRIM_BLOCK, min(p1, p0 + k PRIM_BLOCK)).
+#define PRIM_BLOCK 8
+#define PAIR_MASK 0xFFFFFFu
+__host__ __device__ __forceinline__ unsigned pair_of(unsigned code) { return code & PAIR_MASK; }
+__host__ __device__ __forceinline__ unsigned block_of(unsigned code) { return code >> 24; }
+__host__ __device__ __forceinline__ void prim_range(const PairInfo& P, unsigned blk, int& p0, int& p1) {
+  if (blk == 0) {
+    p0 = P.p0;
+    p1 = P.p1;
+  } else {
+    p0 = P.p0 + (int)(blk - 1) * PRIM_BLOCK;
+    p1 = P.p1 < p0 + PRIM_BLOCK ? P.p1 : p0 + PRIM_BLOCK;
+  }
+}
 
 // Wavefront-reduced add into LDS (every lane targets the same LDS word).
 __device__ __forceinline__ void wave_add_lds(double* p, double v) {

@@ -12,15 +12,19 @@ __device__ __forceinline__ void make_quartet(const uint2* __restrict__ tasks, lo
                                              const PrimPair* __restrict__ prims, QuartetArgs& q) {
   const bool active = t < count;
   const uint2 tk = tasks[active ? t : wbase];
-  const PairInfo B = pairs[tk.x];
-  const PairInfo K = pairs[tk.y];
+  const unsigned bx = pair_of(tk.x), ky = pair_of(tk.y);
+  const PairInfo B = pairs[bx];
+  const PairInfo K = pairs[ky];
+  int bp0, bp1, kp0, kp1;
+  prim_range(B, block_of(tk.x), bp0, bp1);
+  prim_range(K, block_of(tk.y), kp0, kp1);
   q.prim = prims;
-  q.bra_p0 = B.p0;
-  q.bra_p1 = active ? B.p1 : B.p0;
-  q.ket_p0 = K.p0;
-  q.ket_p1 = K.p1;
+  q.bra_p0 = bp0;
+  q.bra_p1 = active ? bp1 : bp0;
+  q.ket_p0 = kp0;
+  q.ket_p1 = kp1;
   q.a0 = B.a0; q.b0 = B.b0; q.c0 = K.a0; q.d0 = K.b0;
-  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
+  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (bx == ky ? 1.0 : 2.0);
   q.deg = active ? deg : 0.0;
   const unsigned int k0 = __shfl((int)tk.y, 0);
   q.ket_uniform = (__ballot(tk.y == k0) == ~0ull);
@@ -28,39 +32,49 @@ __device__ __forceinline__ void make_quartet(const uint2* __restrict__ tasks, lo
   q.ket_swap = K.a != K.s1;
 }
 
-// Quartet of one lane's bra pair B with the wavefront-uniform ket pair kk (kk must come from
-// readfirstlane so the ket data is loaded through the scalar path); lanes that do not own a
-// task of this ket get an empty bra range and degeneracy 0.
-__device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsigned bra_idx, unsigned kk, bool mine,
+// Quartet of one lane's bra (pair B, task code bra_code) with the wavefront-uniform ket pair
+// kk and its primitive block kblk (both from readfirstlane, so the ket data is loaded through
+// the scalar path); lanes that do not own a task of this ket get an empty bra range and
+// degeneracy 0.
+__device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsigned bra_code, unsigned kk,
+                                                         unsigned kblk, bool mine,
                                                          const PairInfo* __restrict__ pairs,
                                                          const PrimPair* __restrict__ prims, QuartetArgs& q) {
   const PairInfo K = pairs[kk];
+  int bp0, bp1, kp0, kp1;
+  prim_range(B, block_of(bra_code), bp0, bp1);
+  prim_range(K, kblk, kp0, kp1);
   q.prim = prims;
-  q.bra_p0 = B.p0;
-  q.bra_p1 = mine ? B.p1 : B.p0;
-  q.ket_p0 = K.p0;
-  q.ket_p1 = K.p1;
+  q.bra_p0 = bp0;
+  q.bra_p1 = mine ? bp1 : bp0;
+  q.ket_p0 = kp0;
+  q.ket_p1 = kp1;
   q.a0 = B.a0; q.b0 = B.b0; q.c0 = K.a0; q.d0 = K.b0;
-  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (bra_idx == kk ? 1.0 : 2.0);
+  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (pair_of(bra_code) == kk ? 1.0 : 2.0);
   q.deg = mine ? deg : 0.0;
   q.ket_uniform = true;
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
 }
 
-// Quartet of the wavefront-uniform bra pair bb (from readfirstlane) with one lane's ket pair K;
-// lanes that do not own a task get an empty ket range and degeneracy 0.
-__device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsigned ket_idx, unsigned bb, bool mine,
+// Quartet of the wavefront-uniform bra pair bb and its primitive block bblk (from
+// readfirstlane) with one lane's ket (pair K, task code ket_code); lanes that do not own a task
+// get an empty ket range and degeneracy 0.
+__device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsigned ket_code, unsigned bb,
+                                                         unsigned bblk, bool mine,
                                                          const PairInfo* __restrict__ pairs,
                                                          const PrimPair* __restrict__ prims, QuartetArgs& q) {
   const PairInfo B = pairs[bb];
+  int bp0, bp1, kp0, kp1;
+  prim_range(B, bblk, bp0, bp1);
+  prim_range(K, block_of(ket_code), kp0, kp1);
   q.prim = prims;
-  q.bra_p0 = B.p0;
-  q.bra_p1 = B.p1;
-  q.ket_p0 = K.p0;
-  q.ket_p1 = mine ? K.p1 : K.p0;
+  q.bra_p0 = bp0;
+  q.bra_p1 = bp1;
+  q.ket_p0 = kp0;
+  q.ket_p1 = mine ? kp1 : kp0;
   q.a0 = B.a0; q.b0 = B.b0; q.c0 = K.a0; q.d0 = K.b0;
-  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (bb == ket_idx ? 1.0 : 2.0);
+  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (bb == pair_of(ket_code) ? 1.0 : 2.0);
   q.deg = mine ? deg : 0.0;
   q.ket_uniform = false;
   q.bra_swap = B.a != B.s1;
@@ -89,7 +103,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       const long t = tbase + (threadIdx.x & 63);                                                \
       const bool active = t < count;                                                            \
       const uint2 tk = tasks[active ? t : tbase];                                               \
-      const PairInfo B = pairs[tk.x];                                                           \
+      const PairInfo B = pairs[pair_of(tk.x)];                                                  \
       bool pending = active;                                                                    \
       /* waterfall over the distinct ket pairs of the wavefront: each pass has one uniform ket \
          (scalar loads of its primitive data), lanes of other kets contribute nothing */        \
@@ -100,7 +114,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
         const unsigned kk = __builtin_amdgcn_readfirstlane(__shfl((int)tk.y, leader));          \
         const bool mine = pending && tk.y == kk;                                                \
         QuartetArgs q;                                                                          \
-        make_quartet_uniform_ket(B, tk.x, kk, mine, pairs, prims, q);                           \
+        make_quartet_uniform_ket(B, tk.x, kk, 0u, mine, pairs, prims, q);                       \
         quartet_jk_##NAME<NSLOT, 0>(q, jk, boys_lds, chunk, nullptr, nullptr);                  \
         pending = pending && !mine;                                                             \
       }                                                                                         \
@@ -135,14 +149,15 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       const Segment sg = segs[item / NCH];                                                    \
       const int chunk = (int)(item % NCH);                                                    \
       const unsigned kk = __builtin_amdgcn_readfirstlane((unsigned)sg.ket);                   \
+      const unsigned kblk = __builtin_amdgcn_readfirstlane((unsigned)sg.blk);                 \
       for (int t0 = 0; t0 < sg.count; t0 += WG) {                                             \
         if (t0 + wofs >= sg.count) continue;                                                  \
         const int t = t0 + threadIdx.x;                                                       \
         const bool active = t < sg.count;                                                     \
         const uint2 tk = tasks[sg.off + (active ? t : t0 + wofs)];                            \
-        const PairInfo B = pairs[tk.x];                                                       \
+        const PairInfo B = pairs[pair_of(tk.x)];                                              \
         QuartetArgs q;                                                                        \
-        make_quartet_uniform_ket(B, tk.x, kk, active, pairs, prims, q);                       \
+        make_quartet_uniform_ket(B, tk.x, kk, kblk, active, pairs, prims, q);                 \
         quartet_jk_##NAME<1, 1>(q, jk, boys_lds, chunk, colK, jcdL);                          \
       }                                                                                       \
       __syncthreads();                                                                        \
@@ -216,14 +231,15 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       const Segment sg = segs[item / NCH];                                                      \
       const int chunk = (int)(item % NCH);                                                      \
       const unsigned bb = __builtin_amdgcn_readfirstlane((unsigned)sg.ket);                     \
+      const unsigned bblk = __builtin_amdgcn_readfirstlane((unsigned)sg.blk);                   \
       for (int t0 = 0; t0 < sg.count; t0 += WG) {                                               \
         if (t0 + wofs >= sg.count) continue;                                                    \
         const int t = t0 + threadIdx.x;                                                         \
         const bool active = t < sg.count;                                                       \
         const uint2 tk = tasks[sg.off + (active ? t : t0 + wofs)];                              \
-        const PairInfo K = pairs[tk.y];                                                         \
+        const PairInfo K = pairs[pair_of(tk.y)];                                                \
         QuartetArgs q;                                                                          \
-        make_quartet_uniform_bra(K, tk.y, bb, active, pairs, prims, q);                         \
+        make_quartet_uniform_bra(K, tk.y, bb, bblk, active, pairs, prims, q);                   \
         quartet_jk_##NAME<1, 2>(q, jk, boys_lds, chunk, rowK, jabL);                            \
       }                                                                                         \
       __syncthreads();                                                                          \
