@@ -77,18 +77,21 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     const int2* __restrict__ s12, const double* __restrict__ pairQ, const int* __restrict__ grp,
     const int* __restrict__ cls_grp_off, const double* __restrict__ Nmax,
     const unsigned long long* __restrict__ nglob_bits, int nshell, double tau,
-    const unsigned char* __restrict__ pair_nblk, uint2* __restrict__ tasks,
+    const unsigned char* __restrict__ pair_nblk, const unsigned char* __restrict__ pair_nublk,
+    uint2* __restrict__ tasks,
     unsigned long long* __restrict__ counts, unsigned long long* __restrict__ qcounts,
-    Segment* __restrict__ segs, unsigned* __restrict__ nseg) {
+    Segment* __restrict__ segs, unsigned* __restrict__ nseg, int cp_fixed) {
   __shared__ uint2 buf[WG / 64][SCREEN_BUF];
   const int wid = threadIdx.x >> 6;
   const int wave = blockIdx.x * (WG / 64) + wid;
   const int lane = threadIdx.x & 63;
   if (wave >= nitem) return;
-  int cp = 0;
-  while (cp + 1 < ncp && item_start[cp + 1] <= wave) ++cp;
+  // cp_fixed >= 0: this launch screens that class pair only (nitem = its uniform pairs)
+  int cp = cp_fixed >= 0 ? cp_fixed : 0;
+  if (cp_fixed < 0)
+    while (cp + 1 < ncp && item_start[cp + 1] <= wave) ++cp;
   const CPMeta m = cps[cp];
-  const int ket_local = wave - item_start[cp];
+  const int ket_local = cp_fixed >= 0 ? wave : wave - item_start[cp];
   const int K = (m.flip ? m.bra_start : m.ket_start) + ket_local;   // the uniform pair
   const int sw_start = m.flip ? m.ket_start : m.bra_start;           // the swept class
   const int nsw = m.flip ? m.nket : m.nbra;
@@ -101,8 +104,8 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   uint2* wbuf = buf[wid];
   int nb = 0;
   unsigned long long nq = 0;
-  // a uniform pair with more than PRIM_BLOCK primitive pairs gets one segment per block
-  const int ublk = pair_nblk[K];
+  // a uniform pair with more than UNI_BLOCK primitive pairs gets one segment per block
+  const int ublk = pair_nublk[K];
   auto flush = [&]() {
     wave_sync_lds();
     unsigned long long base = 0;
@@ -205,20 +208,18 @@ __global__ void norm_kernel(MatList ml, int ld, const ShellMeta* __restrict__ sh
   atomicMax(nglob_bits, (unsigned long long)__double_as_longlong(mx));
 }
 
-// flag = 1 when X is not exactly symmetric.
-__global__ void nonsym_kernel(const double* __restrict__ X, int n, int ld, int* flag) {
-  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (long)n * n) return;
-  const int i = k % n, j = k / n;
-  if (i < j && X[(long)j * ld + i] != X[(long)i * ld + j]) atomicOr(flag, 1);
-}
 
 // Cartesian density (row-major ncart x ncart): Dc = T^T X T.
-__global__ void cart_density_kernel(const double* __restrict__ X, int ld, const ShellMeta* __restrict__ sh,
+// D_cart = T^T X T for one density; threads below n*n also flag a non-symmetric X.
+__global__ void cart_density_kernel(const double* __restrict__ X, int n, int ld, const ShellMeta* __restrict__ sh,
                                     const int* __restrict__ cart2shell, int ncart,
                                     const double* __restrict__ tco, const int* __restrict__ tco_off,
-                                    double* __restrict__ Dc) {
+                                    double* __restrict__ Dc, int* __restrict__ nonsym) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < (long)n * n) {
+    const int i = k % n, j = k / n;
+    if (i < j && X[(long)j * ld + i] != X[(long)i * ld + j]) atomicOr(nonsym, 1);
+  }
   if (k >= (long)ncart * ncart) return;
   const int a = k / ncart, b = k - (long)a * ncart;
   const int sa = cart2shell[a], sb = cart2shell[b];
@@ -239,31 +240,56 @@ __global__ void cart_density_kernel(const double* __restrict__ X, int ld, const 
   Dc[k] = sk;
 }
 
-// AX[mu + nu*ld] += f * sum_{a,c} T(mu,a) T(nu,c) (G1[a][c] + G2[c][a])
-__global__ void finalize_kernel(double* __restrict__ AX, int n, int ld, double f,
-                                const double* __restrict__ G1, const double* __restrict__ G2,
-                                const int* __restrict__ nonsym, const double* __restrict__ G2alt,
-                                const ShellMeta* __restrict__ sh, const int* __restrict__ sph2shell, int ncart,
+// AX_i[mu + nu*ld] += f_i * sum_{a,c} T(mu,a) T(nu,c) (G1_i[a][c] + G2_i[c][a]) for every
+// matrix i of the call, G2_i = G1_i unless the density was flagged non-symmetric.
+struct FinList {
+  double* AX[MAX_MATS];
+  const double* G1[MAX_MATS];
+  const double* G2alt[MAX_MATS];
+  const int* nonsym[MAX_MATS];
+  double f[MAX_MATS];
+  int n;
+};
+__global__ void finalize_kernel(FinList fl, int n, int ld, const ShellMeta* __restrict__ sh,
+                                const int* __restrict__ sph2shell, int ncart,
                                 const double* __restrict__ tco, const int* __restrict__ tco_off) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)n * n) return;
   const int mu = k % n, nu = k / n;
-  const double* g2 = (nonsym && *nonsym) ? G2alt : G2;
   const ShellMeta A = sh[sph2shell[mu]], B = sh[sph2shell[nu]];
   const int im = mu - A.sph_off, in = nu - B.sph_off;
   const int nca = (A.l + 1) * (A.l + 2) / 2, ncb = (B.l + 1) * (B.l + 2) / 2;
-  double s = 0.0;
-  for (int a = 0; a < nca; ++a) {
-    const double ta = tcoef(A, im, a, tco, tco_off);
-    if (ta == 0.0) continue;
-    for (int c = 0; c < ncb; ++c) {
-      const double tc = tcoef(B, in, c, tco, tco_off);
-      if (tc == 0.0) continue;
-      const long ia = A.cart_off + a, ic = B.cart_off + c;
-      s += ta * tc * (G1[ia * ncart + ic] + g2[ic * ncart + ia]);
+  for (int i = 0; i < fl.n; ++i) {
+    const double* G1 = fl.G1[i];
+    const double* g2 = *fl.nonsym[i] ? fl.G2alt[i] : G1;
+    double s = 0.0;
+    for (int a = 0; a < nca; ++a) {
+      const double ta = tcoef(A, im, a, tco, tco_off);
+      if (ta == 0.0) continue;
+      for (int c = 0; c < ncb; ++c) {
+        const double tc = tcoef(B, in, c, tco, tco_off);
+        if (tc == 0.0) continue;
+        const long ia = A.cart_off + a, ic = B.cart_off + c;
+        s += ta * tc * (G1[ia * ncart + ic] + g2[ic * ncart + ia]);
+      }
     }
+    fl.AX[i][(long)nu * ld + mu] += fl.f[i] * s;
   }
-  AX[(long)nu * ld + mu] += f * s;
+}
+
+// Zero a list of word ranges in one launch (accumulators, counters, flags).
+#define MAX_ZERO 12
+struct ZeroList {
+  unsigned* p[MAX_ZERO];
+  long n[MAX_ZERO];   // 32-bit words
+  int count;
+};
+__global__ void zero_kernel(ZeroList zl) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (int r = 0; r < zl.count; ++r) {
+    unsigned* p = zl.p[r];
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < zl.n[r]; k += stride) p[k] = 0u;
+  }
 }
 
 // out[mu + nu*ld] = sum_{a,c} T(mu,a) T(nu,c) Mc[a][c]   (Cartesian row-major -> shells' basis)
@@ -555,14 +581,14 @@ struct cq_ctx {
   DevBuf<unsigned> d_nseg, d_work_q;
   int lds_mode = 2;   // 0 HBM atomics, 1 ket-keyed LDS segments, 2 also bra-keyed segments
   DevBuf<unsigned long long> d_counts, d_qcounts, d_nglob;
-  DevBuf<unsigned char> d_pair_nblk;
+  DevBuf<unsigned char> d_pair_nblk, d_pair_nublk;
   std::string err;
 
   ~cq_ctx() {
     d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
     d_item_start[0].release(); d_item_start[1].release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
     d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
-    d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_qcounts.release(); d_nglob.release(); d_pair_nblk.release();
+    d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_qcounts.release(); d_nglob.release(); d_pair_nblk.release(); d_pair_nublk.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < kAux; ++k) {
@@ -792,21 +818,23 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     c->cps[t].clear();
     c->item_start[t].clear();
   }
-  // primitive blocks per pair (1: unsplit)
-  std::vector<unsigned char> nblk(npair);
+  // primitive blocks per pair (1: unsplit) as the per-lane side and as the uniform side
+  std::vector<unsigned char> nblk(npair), nublk(npair);
   for (int k = 0; k < npair; ++k) {
     const int np = c->pairs[k].p1 - c->pairs[k].p0;
     if (np > 64 * PRIM_BLOCK) throw std::invalid_argument("more than 512 primitive pairs in one shell pair");
     nblk[k] = (unsigned char)(np > PRIM_BLOCK ? (np + PRIM_BLOCK - 1) / PRIM_BLOCK : 1);
+    nublk[k] = (unsigned char)(np > UNI_BLOCK ? (np + UNI_BLOCK - 1) / UNI_BLOCK : 1);
   }
   if ((unsigned)npair > PAIR_MASK) throw std::invalid_argument("too many shell pairs");
   c->d_pair_nblk.upload(nblk, c->stream);
-  auto class_blocks = [&](int cls, long long& sum, int& mx) {
+  c->d_pair_nublk.upload(nublk, c->stream);
+  auto class_blocks = [&](int cls, const std::vector<unsigned char>& nb, long long& sum, int& mx) {
     sum = 0;
     mx = 1;
     for (int k = c->class_start[cls]; k < c->class_start[cls + 1]; ++k) {
-      sum += nblk[k];
-      mx = std::max<int>(mx, nblk[k]);
+      sum += nb[k];
+      mx = std::max<int>(mx, nb[k]);
     }
   };
   long long off = 0, seg_total[2] = {0, 0};
@@ -821,10 +849,12 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     m.same = ck.i == ck.j;
     m.bra_class = ck.bi;
     m.ket_class = ck.ki;
-    long long bsum, ksum;
-    int bmax, kmax;
-    class_blocks(ck.bi, bsum, bmax);
-    class_blocks(ck.ki, ksum, kmax);
+    long long bsum, ksum, busum, kusum;
+    int bmax, kmax, bumax, kumax;
+    class_blocks(ck.bi, nblk, bsum, bmax);
+    class_blocks(ck.ki, nblk, ksum, kmax);
+    class_blocks(ck.bi, nublk, busum, bumax);
+    class_blocks(ck.ki, nublk, kusum, kumax);
     // task entries: every swept pair's blocks against every uniform pair (same class: bra >= ket)
     long long cap_ket = 0;
     if (m.same) {
@@ -845,7 +875,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
       // segments: a uniform pair with T tasks is flushed at most T / (SCREEN_BUF - 64) + 1 times
       // (a flush happens once fewer than 64 slots are left), each flush writing one segment per
       // primitive block of the uniform pair
-      mt.seg_cap = (mt.flip ? bsum : ksum) + (mt.task_cap / (SCREEN_BUF - 64) + 1) * (mt.flip ? bmax : kmax);
+      mt.seg_cap = (mt.flip ? busum : kusum) + (mt.task_cap / (SCREEN_BUF - 64) + 1) * (mt.flip ? bumax : kumax);
       mt.seg_off = seg_total[t];
       seg_total[t] += mt.seg_cap;
       c->cps[t].push_back(mt);
@@ -886,7 +916,7 @@ struct Slot {
 // The contraction on device-resident X/AX (column-major, leading dimension ld).
 void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const* AX, const int* types,
                      const double* scales, int n, int ld, hipStream_t s) {
-  if (nmat < 1) throw std::invalid_argument("nmat must be >= 1");
+  if (nmat < 1 || nmat > MAX_MATS) throw std::invalid_argument("nmat must be in 1..8");
   if (n != c->nsph) throw std::invalid_argument("matrix dimension does not match the basis");
   if (ld < n) throw std::invalid_argument("leading dimension smaller than the matrix dimension");
   for (int i = 0; i < nmat; ++i) {
@@ -915,45 +945,45 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   // workspace: per slot Dc, GJ, GJ2, GK, GK2
   c->d_work.ensure((size_t)std::max(1, nslot) * 5 * nc2);
   c->d_flags.ensure(std::max(1, nslot));
-  // 1. block norms over every input matrix
+  // 1. one launch zeroes the accumulators, counters and flags
+  ZeroList zl{};
+  auto zero = [&](void* p, size_t bytes) {
+    zl.p[zl.count] = (unsigned*)p;
+    zl.n[zl.count] = (long)(bytes / 4);
+    ++zl.count;
+  };
+  zero(c->d_nglob.p, sizeof(unsigned long long));
+  zero(c->d_flags.p, std::max(1, nslot) * sizeof(int));
+  zero(c->d_counts.p, kNumClassPairs * sizeof(unsigned long long));
+  zero(c->d_qcounts.p, kNumClassPairs * sizeof(unsigned long long));
+  zero(c->d_nseg.p, kNumClassPairs * sizeof(unsigned));
+  zero(c->d_work_q.p, kNumClassPairs * sizeof(unsigned));
+  // the whole workspace (the Cartesian densities in it are rewritten below)
+  zero(c->d_work.p, (size_t)std::max(1, nslot) * 5 * nc2 * sizeof(double));
+  hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(WG), 0, s, zl);
+  HIPCHK(hipGetLastError());
+  // 2. block norms over every input matrix (scale 0 included)
   MatList ml{};
   ml.n = (int)uniq.size();
   for (int i = 0; i < ml.n; ++i) ml.X[i] = uniq[i];
-  HIPCHK(hipMemsetAsync(c->d_nglob.p, 0, sizeof(unsigned long long), s));
   hipLaunchKernelGGL(norm_kernel, dim3(grid_for((long)ns * ns, WG)), dim3(WG), 0, s, ml, ld, c->d_shells.p, ns,
                      c->d_Nmax.p, c->d_nglob.p);
   HIPCHK(hipGetLastError());
   if (nslot == 0) return;
-  // 2. Cartesian densities, symmetry flags, zeroed accumulators
-  //    per slot: [Dc | GJ | GJ2 | GK | GK2], each ncart^2
-  HIPCHK(hipMemsetAsync(c->d_flags.p, 0, nslot * sizeof(int), s));
+  // 3. Cartesian densities and symmetry flags; per slot: [Dc | GJ | GJ2 | GK | GK2], ncart^2 each
   for (int k = 0; k < nslot; ++k) {
     double* base = c->d_work.p + (size_t)k * 5 * nc2;
-    hipLaunchKernelGGL(cart_density_kernel, dim3(grid_for(nc2, WG)), dim3(WG), 0, s, slots[k].X, ld,
-                       c->d_shells.p, c->d_cart2shell.p, c->ncart, c->d_tco.p, c->d_tco_off.p, base);
+    hipLaunchKernelGGL(cart_density_kernel, dim3(grid_for(std::max<long>(nc2, (long)n * n), WG)), dim3(WG), 0, s,
+                       slots[k].X, n, ld, c->d_shells.p, c->d_cart2shell.p, c->ncart, c->d_tco.p, c->d_tco_off.p,
+                       base, c->d_flags.p + k);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(nonsym_kernel, dim3(grid_for((long)n * n, WG)), dim3(WG), 0, s, slots[k].X, n, ld,
-                       c->d_flags.p + k);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemsetAsync(base + nc2, 0, 4 * nc2 * sizeof(double), s));
   }
-  // 3. screening into compacted task lists
-  HIPCHK(hipMemsetAsync(c->d_counts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
-  HIPCHK(hipMemsetAsync(c->d_qcounts.p, 0, kNumClassPairs * sizeof(unsigned long long), s));
-  HIPCHK(hipMemsetAsync(c->d_nseg.p, 0, kNumClassPairs * sizeof(unsigned), s));
-  HIPCHK(hipMemsetAsync(c->d_work_q.p, 0, kNumClassPairs * sizeof(unsigned), s));
   // bra-keyed segments need the segment kernels for every pass: one density slot, LDS on
   const int tb = (c->lds_mode >= 2 && nslot == 1) ? 1 : 0;
   const std::vector<CPMeta>& cps = c->cps[tb];
-  if (c->nitem[tb] > 0) {
-    hipLaunchKernelGGL(screen_kernel, dim3(grid_for(c->nitem[tb], WG / 64)), dim3(WG), 0, s, c->d_cps[tb].p,
-                       kNumClassPairs, c->d_item_start[tb].p, c->nitem[tb], c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
-                       c->d_cls_grp_off.p, c->d_Nmax.p,
-                       c->d_nglob.p, ns, c->tau, c->d_pair_nblk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p,
-                       c->d_segs.p, c->d_nseg.p);
-    HIPCHK(hipGetLastError());
-  }
-  // 4. ERI + J/K digestion, class pair by class pair, two density slots per pass
+  // 4. per class pair (on the auxiliary streams, highest angular momentum first: those are the
+  //    latency-bound ones): screening into its compacted task list, then ERI + J/K digestion,
+  //    two density slots per pass; joined back into s before the finalisation
   for (int s0 = 0; s0 < nslot; s0 += MAX_SLOTS) {
     const int ng = std::min(MAX_SLOTS, nslot - s0);
     JKArgs jk{};
@@ -969,8 +999,6 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       jk.do_k[g] = slots[s0 + g].do_k;
     }
     jk.nonsym = c->d_flags.p + s0;
-    // class-pair kernels spread over the auxiliary streams (highest angular momentum first:
-    // those are the latency-bound ones), joined back into s before the finalisation
     HIPCHK(hipEventRecord(c->ev_fork, s));
     for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
     int launched = 0;
@@ -980,6 +1008,14 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const CPMeta& m = cps[k];
       if (m.nbra == 0 || m.nket == 0) continue;
       hipStream_t sk = c->aux[launched++ % c->n_streams];
+      if (s0 == 0) {
+        const int nuni = m.flip ? m.nbra : m.nket;
+        hipLaunchKernelGGL(screen_kernel, dim3(grid_for(nuni, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p,
+                           kNumClassPairs, c->d_item_start[tb].p, nuni, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
+                           c->d_cls_grp_off.p, c->d_Nmax.p, c->d_nglob.p, ns, c->tau, c->d_pair_nblk.p,
+                           c->d_pair_nublk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p, c->d_segs.p, c->d_nseg.p, k);
+        HIPCHK(hipGetLastError());
+      }
       const uint2* tasks = c->d_tasks.p + m.task_off;
       const long long cap = m.task_cap;
       const long long items = ((cap + 63) / 64) * 64 * ck.nchunk;
@@ -1002,8 +1038,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
                            c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
       } else if (seg_path) {
-        const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
-        hipLaunchKernelGGL(ck.jkseg, dim3(std::max(1, sgrid)), dim3(WG), lds_bytes, sk, tasks,
+        const int sgrid = (int)std::min<long long>((long long)persistent_grid() * WG / ck.seg_wg, m.seg_cap * ck.nchunk);
+        hipLaunchKernelGGL(ck.jkseg, dim3(std::max(1, sgrid)), dim3(ck.seg_wg), lds_bytes, sk, tasks,
                            c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
       } else {
@@ -1021,18 +1057,23 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       HIPCHK(hipStreamWaitEvent(s, c->ev_join[a], 0));
     }
   }
-  // 5. back to the shells' basis and accumulate: AX += 1/2 scale (G + G^T-partner)
+  // 5. back to the shells' basis and accumulate, every matrix in one launch:
+  //    AX += 1/2 scale (G + G'^T); COULOMB G = GJ, G' = (nonsym ? GJ2 : GJ); EXCHANGE with GK/GK2
+  FinList fl{};
   for (int i = 0; i < nmat; ++i) {
     const int k = mat_slot[i];
     if (k < 0) continue;
     double* base = c->d_work.p + (size_t)k * 5 * nc2;
-    const double f = 0.5 * scales[i];
-    // COULOMB: G = GJ + (nonsym ? GJ2 : GJ)^T ; EXCHANGE: G = GK + (nonsym ? GK2 : GK)^T
-    const double* G1 = base + (types[i] == CQ_COULOMB ? 1 : 3) * nc2;
-    const double* G2 = base + (types[i] == CQ_COULOMB ? 2 : 4) * nc2;
-    hipLaunchKernelGGL(finalize_kernel, dim3(grid_for((long)n * n, WG)), dim3(WG), 0, s, AX[i], n, ld, f, G1, G1,
-                       c->d_flags.p + k, G2, c->d_shells.p, c->d_sph2shell.p, c->ncart, c->d_tco.p,
-                       c->d_tco_off.p);
+    fl.AX[fl.n] = AX[i];
+    fl.G1[fl.n] = base + (types[i] == CQ_COULOMB ? 1 : 3) * nc2;
+    fl.G2alt[fl.n] = base + (types[i] == CQ_COULOMB ? 2 : 4) * nc2;
+    fl.nonsym[fl.n] = c->d_flags.p + k;
+    fl.f[fl.n] = 0.5 * scales[i];
+    ++fl.n;
+  }
+  if (fl.n > 0) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(grid_for((long)n * n, WG)), dim3(WG), 0, s, fl, n, ld, c->d_shells.p,
+                       c->d_sph2shell.p, c->ncart, c->d_tco.p, c->d_tco_off.p);
     HIPCHK(hipGetLastError());
   }
 }

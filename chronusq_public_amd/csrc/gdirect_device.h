@@ -75,40 +75,50 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 // e^-T term is below 2.4e-16).
 template <int L>
 __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ lds, double* F) {
-  if (T < BOYS_TMAX) {
-    const int i = (int)(T * BOYS_INV_STEP + 0.5);
-    const double dT = (double)i * BOYS_STEP - T;
-    const double* f = lds + i * BOYS_LDS_W;
-    double s = f[7];
-    s = f[6] + s * dT * (1.0 / 7.0);
-    s = f[5] + s * dT * (1.0 / 6.0);
-    s = f[4] + s * dT * (1.0 / 5.0);
-    s = f[3] + s * dT * (1.0 / 4.0);
-    s = f[2] + s * dT * (1.0 / 3.0);
-    s = f[1] + s * dT * (1.0 / 2.0);
-    s = f[0] + s * dT;
-    F[L] = s;
-    if constexpr (L > 0) {
-      double e = 1.0 + dT * (1.0 / 8.0);
-      e = 1.0 + dT * (1.0 / 7.0) * e;
-      e = 1.0 + dT * (1.0 / 6.0) * e;
-      e = 1.0 + dT * (1.0 / 5.0) * e;
-      e = 1.0 + dT * (1.0 / 4.0) * e;
-      e = 1.0 + dT * (1.0 / 3.0) * e;
-      e = 1.0 + dT * (1.0 / 2.0) * e;
-      e = 1.0 + dT * e;
-      e *= f[8];
-      const double T2 = 2.0 * T;
-#pragma unroll
-      for (int m = L - 1; m >= 0; --m) F[m] = (T2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
-    }
+  // Both branches are evaluated and the result selected, so that the primitive loop has no
+  // divergent branch (unrolled iterations then interleave): the table path at min(T, T_max)
+  // (index always in range), the asymptotic path at max(T, T_max).
+  const bool big = T >= BOYS_TMAX;
+  const double Tt = big ? 0.0 : T;
+  const int i = (int)(Tt * BOYS_INV_STEP + 0.5);
+  const double dT = (double)i * BOYS_STEP - Tt;
+  const double* f = lds + i * BOYS_LDS_W;
+  double s = f[7];
+  s = f[6] + s * dT * (1.0 / 7.0);
+  s = f[5] + s * dT * (1.0 / 6.0);
+  s = f[4] + s * dT * (1.0 / 5.0);
+  s = f[3] + s * dT * (1.0 / 4.0);
+  s = f[2] + s * dT * (1.0 / 3.0);
+  s = f[1] + s * dT * (1.0 / 2.0);
+  s = f[0] + s * dT;
+  const double Ta = big ? T : BOYS_TMAX;
+  const double rs = rsqrt_f64(Ta);
+  double fa = 0.88622692545275801365 * rs;   // sqrt(pi)/2 T^-1/2
+  if constexpr (L == 0) {
+    F[0] = big ? fa : s;
   } else {
-    const double rs = rsqrt_f64(T);
-    F[0] = 0.88622692545275801365 * rs;   // sqrt(pi)/2
-    if constexpr (L > 0) {
-      const double hr = 0.5 * rs * rs;
+    // table: F_L by Taylor, then downward recursion with e^{-T} (Taylor about T_i)
+    double e = 1.0 + dT * (1.0 / 8.0);
+    e = 1.0 + dT * (1.0 / 7.0) * e;
+    e = 1.0 + dT * (1.0 / 6.0) * e;
+    e = 1.0 + dT * (1.0 / 5.0) * e;
+    e = 1.0 + dT * (1.0 / 4.0) * e;
+    e = 1.0 + dT * (1.0 / 3.0) * e;
+    e = 1.0 + dT * (1.0 / 2.0) * e;
+    e = 1.0 + dT * e;
+    e *= f[8];
+    const double T2 = 2.0 * Tt;
+    double Ft[L + 1];
+    Ft[L] = s;
 #pragma unroll
-      for (int m = 0; m < L; ++m) F[m + 1] = F[m] * (double)(2 * m + 1) * hr;
+    for (int m = L - 1; m >= 0; --m) Ft[m] = (T2 * Ft[m + 1] + e) * (1.0 / (2 * m + 1));
+    // asymptotic: F_0 = sqrt(pi/T)/2, upward F_{m+1} = F_m (2m+1) / (2T)
+    const double hr = 0.5 * rs * rs;
+    F[0] = big ? fa : Ft[0];
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      fa = fa * (double)(2 * m + 1) * hr;
+      F[m + 1] = big ? fa : Ft[m + 1];
     }
   }
 }
@@ -128,22 +138,26 @@ struct Segment {
   int off, count, ket, blk;   // ket: the uniform pair; blk: its primitive-block code
 };
 
-// Primitive blocks.  A pair with more than PRIM_BLOCK primitive pairs is split into blocks of
-// PRIM_BLOCK so that one heavily contracted quartet becomes several shorter lane tasks (the
-// digestion is linear in the integrals, so partial sums digest independently).  Task entries
-// and segments carry a block code in bits 24..31 of the pair index: 0 = the whole primitive
-// range, k >= 1 = primitives [p0 + (k-1) PRIM_BLOCK, min(p1, p0 + k PRIM_BLOCK)).
+// Primitive blocks.  A pair with many primitive pairs is split into blocks so that one heavily
+// contracted quartet becomes several shorter lane tasks (the digestion is linear in the
+// integrals, so partial sums digest independently): blocks of PRIM_BLOCK on the per-lane side
+// of a task list, blocks of UNI_BLOCK on the wavefront-uniform side (each uniform block is a
+// separate work item that repeats the per-lane J atomics and the LDS flush, so it is kept
+// coarse).  Task entries and segments carry a block code in bits 24..31 of the pair index:
+// 0 = the whole primitive range, k >= 1 = primitives [p0 + (k-1) B, min(p1, p0 + k B)).
 #define PRIM_BLOCK 8
+#define UNI_BLOCK 32
 #define PAIR_MASK 0xFFFFFFu
 __host__ __device__ __forceinline__ unsigned pair_of(unsigned code) { return code & PAIR_MASK; }
 __host__ __device__ __forceinline__ unsigned block_of(unsigned code) { return code >> 24; }
+template <int BLK = PRIM_BLOCK>
 __host__ __device__ __forceinline__ void prim_range(const PairInfo& P, unsigned blk, int& p0, int& p1) {
   if (blk == 0) {
     p0 = P.p0;
     p1 = P.p1;
   } else {
-    p0 = P.p0 + (int)(blk - 1) * PRIM_BLOCK;
-    p1 = P.p1 < p0 + PRIM_BLOCK ? P.p1 : p0 + PRIM_BLOCK;
+    p0 = P.p0 + (int)(blk - 1) * BLK;
+    p1 = P.p1 < p0 + BLK ? P.p1 : p0 + BLK;
   }
 }
 

@@ -43,7 +43,7 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
   const PairInfo K = pairs[kk];
   int bp0, bp1, kp0, kp1;
   prim_range(B, block_of(bra_code), bp0, bp1);
-  prim_range(K, kblk, kp0, kp1);
+  prim_range<UNI_BLOCK>(K, kblk, kp0, kp1);
   q.prim = prims;
   q.bra_p0 = bp0;
   q.bra_p1 = mine ? bp1 : bp0;
@@ -66,7 +66,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
                                                          const PrimPair* __restrict__ prims, QuartetArgs& q) {
   const PairInfo B = pairs[bb];
   int bp0, bp1, kp0, kp1;
-  prim_range(B, bblk, bp0, bp1);
+  prim_range<UNI_BLOCK>(B, bblk, bp0, bp1);
   prim_range(K, block_of(ket_code), kp0, kp1);
   q.prim = prims;
   q.bra_p0 = bp0;
@@ -120,10 +120,11 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       }                                                                                         \
     }                                                                                           \
   }                                                                                             \
-  /* Segment kernel (one density slot): each workgroup takes (segment, chunk) items from a  \
-     queue; the segment's ket is uniform, K columns c/d and J(3,4) accumulate in LDS and are \
+  /* Segment kernel (one density slot, WGS threads per workgroup): each workgroup takes     \
+     (segment, chunk) items from a queue; the segment's ket is uniform, K columns c/d and J(3,4) accumulate in LDS and are \
      flushed to HBM once per item. */                                                        \
-  __global__ void __launch_bounds__(WG) eri_jkseg_##NAME(                                     \
+  template <int WGS>                                                                          \
+  __global__ void __launch_bounds__(WGS) eri_jkseg_##NAME(                                     \
       const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                      \
       const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                     \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                 \
@@ -138,7 +139,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
     const int N = jk.ncart;                                                                   \
     double* colK = dyn_lds;                                                                   \
     double* jcdL = dyn_lds + NCOL * N;                                                        \
-    for (int k = threadIdx.x; k < NCOL * N + NC * ND; k += WG) dyn_lds[k] = 0.0;             \
+    for (int k = threadIdx.x; k < NCOL * N + NC * ND; k += WGS) dyn_lds[k] = 0.0;             \
     const int wofs = threadIdx.x & ~63;                                                       \
     while (true) {                                                                            \
       __syncthreads();                                                                        \
@@ -150,7 +151,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       const int chunk = (int)(item % NCH);                                                    \
       const unsigned kk = __builtin_amdgcn_readfirstlane((unsigned)sg.ket);                   \
       const unsigned kblk = __builtin_amdgcn_readfirstlane((unsigned)sg.blk);                 \
-      for (int t0 = 0; t0 < sg.count; t0 += WG) {                                             \
+      for (int t0 = 0; t0 < sg.count; t0 += WGS) {                                             \
         if (t0 + wofs >= sg.count) continue;                                                  \
         const int t = t0 + threadIdx.x;                                                       \
         const bool active = t < sg.count;                                                     \
@@ -162,7 +163,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       }                                                                                       \
       __syncthreads();                                                                        \
       const PairInfo K = pairs[kk];                                                           \
-      for (int k = threadIdx.x; k < NCOL * N; k += WG) {                                      \
+      for (int k = threadIdx.x; k < NCOL * N; k += WGS) {                                      \
         const double v = colK[k];                                                             \
         if (v != 0.0) {                                                                       \
           const int col = k / N, row = k - col * N;                                           \
@@ -280,6 +281,7 @@ struct ClassKernels {
   SegKernel jkseg;
   SegKernel jksegb;   // bra-uniform segment kernel, or null
   int nc, nd, na, nb;
+  int seg_wg;         // workgroup size of jkseg
 };
 
 

@@ -586,10 +586,14 @@ def main(out_dir: str, maxl: int = 2):
             segb = i != j and (LA, LB) != (0, 0)
             if segb:
                 src.append(f"DEFINE_CLASS_SEGB({name}, {L})")
+            # ket-keyed segments of an (ss) or (ps) bra class hold many bras per ket: 512-thread
+            # workgroups share one Boys table and one set of K columns over 8 wavefronts
+            seg_wg = 512 if (LA, LB) in ((0, 0), (1, 0)) else 256
             src.append(f"void cq_register_{name}(ClassKernels& ck) {{")
             src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, kChunks_{name}, \"{name}\", "
-                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}, "
-                       f"{('eri_jksegb_' + name) if segb else 'nullptr'}, kNC_{name}, kND_{name}, kNA_{name}, kNB_{name}}};")
+                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}<{seg_wg}>, "
+                       f"{('eri_jksegb_' + name) if segb else 'nullptr'}, kNC_{name}, kND_{name}, kNA_{name}, kNB_{name}, "
+                       f"{seg_wg}}};")
             src.append("}")
             _write_if_changed(os.path.join(out_dir, f"eri_{name}.hip"), "\n".join(src) + "\n")
             table.append(name)
