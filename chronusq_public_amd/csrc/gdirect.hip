@@ -1263,9 +1263,17 @@ int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, 
     }
     HIPCHK(hipMemcpyAsync(dZ, zx.data(), zx.size() * sizeof(double), hipMemcpyHostToDevice, s));
     OneEShellData sd{c->d_prim_off.p, c->d_alpha.p, c->d_coef.p, c->d_origin.p};
+    ZeroList zl{};
+    zl.p[0] = (unsigned*)Sc;
+    zl.n[0] = 3 * nc2 * 2;
+    zl.count = 1;
+    hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(WG), 0, s, zl);
+    int max_prim = 1;
+    for (size_t k = 0; k + 1 < c->prim_off_host.size(); ++k)
+      max_prim = std::max(max_prim, c->prim_off_host[k + 1] - c->prim_off_host[k]);
     for (int X = 0; X < c->npc; ++X)
-      launch_onee(X, c->d_pairs.p, c->class_start[X], c->class_start[X + 1], sd, dZ, dxyz, natom, c->d_boys.p,
-                  c->ncart, Sc, Tc, Vc, s);
+      launch_onee(X, c->d_pairs.p, c->class_start[X], c->class_start[X + 1], sd, max_prim, dZ, dxyz, natom,
+                  c->d_boys.p, c->ncart, Sc, Tc, Vc, s);
     HIPCHK(hipGetLastError());
     double* outs[3] = {S, T, V};
     for (int k = 0; k < 3; ++k)

@@ -13,7 +13,8 @@ struct OneEShellData {
 };
 
 // S, T, V Cartesian blocks of the shell pairs [p_begin, p_end) of one pair class
-// (0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd), written into row-major ncart x ncart matrices.
+// (0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd), added into zeroed row-major ncart x ncart matrices;
+// max_prim bounds the primitive count of any shell.
 void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
-                 const double* Z, const double* xyz, int natom, const double* boys, int ncart, double* S, double* T,
-                 double* V, hipStream_t stream);
+                 int max_prim, const double* Z, const double* xyz, int natom, const double* boys, int ncart,
+                 double* S, double* T, double* V, hipStream_t stream);

@@ -6,9 +6,12 @@
 // S_1D(i,j) = E^{ij}_0 sqrt(pi/p),
 // T_1D(i,j) = -1/2 [ j(j-1) S(i,j-2) - 2b(2j+1) S(i,j) + 4b^2 S(i,j+2) ],
 // V = -(2 pi / p) sum_C Z_C sum_tuv E_t E_u E_v R_tuv(p, P - C).
-// One thread per shell pair (l_a >= l_b), every loop unrolled at compile time; S and T in one
-// pass, V in a second pass that first sums R_tuv over the nuclei.  The Cartesian blocks are
-// written to row-major ncart x ncart matrices (and their transposes).
+// One thread per (shell pair, slice), every loop unrolled at compile time: S and T in one pass
+// whose slices are the primitives of shell a, V in a second pass whose slices are groups of
+// ONEE_ATOMS nuclei (summing R_tuv over them first).  Partial blocks are added atomically to
+// zeroed row-major ncart x ncart matrices (and their transposes for a != b).
+#include <algorithm>
+
 #include "gdirect_kernels.h"
 #include "onee.h"
 
@@ -22,8 +25,21 @@ __host__ __device__ constexpr int comp_row(int k) {
 __host__ __device__ constexpr int comp_x(int l, int k) { return l - comp_row(k); }
 __host__ __device__ constexpr int comp_y(int l, int k) { return comp_row(k) - (k - comp_row(k) * (comp_row(k) + 1) / 2); }
 __host__ __device__ constexpr int comp_z(int l, int k) { return l - comp_x(l, k) - comp_y(l, k); }
-__host__ __device__ constexpr double cfact(int n) { return n <= 1 ? 1.0 : n * cfact(n - 1); }
+// n! for n <= 12 from a table (indices are compile-time constants once the loops unroll, so the
+// coefficients fold; a recursive definition would leave run-time calls)
+__host__ __device__ constexpr double cfact(int n) {
+  constexpr double f[13] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0, 362880.0,
+                            3628800.0, 39916800.0, 479001600.0};
+  return f[n < 0 ? 0 : (n > 12 ? 12 : n)];
+}
 __host__ __device__ constexpr double cbinom(int n, int k) { return cfact(n) / (cfact(k) * cfact(n - k)); }
+
+// position of (t, u, v), t + u + v = s, among all Hermite triples with t + u + v <= L:
+// the triples of lower sums first, then by descending t and ascending v
+__host__ __device__ constexpr int hidx(int t, int u, int v) {
+  const int s = t + u + v;
+  return s * (s + 1) * (s + 2) / 6 + (s - t) * (s - t + 1) / 2 + v;
+}
 
 __device__ __forceinline__ double ipow(double x, int n) {
   double r = 1.0;
@@ -52,6 +68,8 @@ __device__ __forceinline__ double hermE(int i, int j, int t, double PA, double P
   return s;
 }
 
+#define ONEE_ATOMS 4
+
 template <int LA, int LB, bool DO_V>
 __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ pairs, int p_begin, int p_end,
                                                   const OneEShellData sd, const double* __restrict__ Zc,
@@ -66,6 +84,12 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
   if (P >= p_end) return;
   const PairInfo pi = pairs[P];
   const int a = pi.a, b = pi.b;
+  // slice: one primitive of a (S, T) or nuclei [c0, c1) (V)
+  const int ip0 = DO_V ? sd.prim_off[a] : sd.prim_off[a] + (int)blockIdx.y;
+  const int ip1 = DO_V ? sd.prim_off[a + 1] : ip0 + 1;
+  if (ip0 >= sd.prim_off[a + 1]) return;
+  const int c0 = DO_V ? (int)blockIdx.y * ONEE_ATOMS : 0;
+  const int c1 = DO_V ? min(natom, c0 + ONEE_ATOMS) : 0;
   const double A[3] = {sd.origin[3 * a], sd.origin[3 * a + 1], sd.origin[3 * a + 2]};
   const double B[3] = {sd.origin[3 * b], sd.origin[3 * b + 1], sd.origin[3 * b + 2]};
   const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
@@ -74,7 +98,7 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
   for (int i = 0; i < NA; ++i)
 #pragma unroll
     for (int j = 0; j < NB; ++j) { s[i][j] = 0.0; t[i][j] = 0.0; }
-  for (int ip = sd.prim_off[a]; ip < sd.prim_off[a + 1]; ++ip)
+  for (int ip = ip0; ip < ip1; ++ip)
     for (int jp = sd.prim_off[b]; jp < sd.prim_off[b + 1]; ++jp) {
       const double al = sd.alpha[ip], be = sd.alpha[jp];
       const double p = al + be, o2p = 0.5 / p;
@@ -103,44 +127,45 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
           }
       } else {
         const double Px = A[0] + PA0, Py = A[1] + PA1, Pz = A[2] + PA2;
-        // Rsum_tuv = sum_C Z_C R_tuv(p, P - C), t + u + w <= L
-        double Rs[L + 1][L + 1][L + 1];
+        // Rsum_tuv = sum_C Z_C R_tuv(p, P - C), t + u + w <= L, in the compact index hidx;
+        // the recursion over n keeps two levels
+        constexpr int NH = (L + 1) * (L + 2) * (L + 3) / 6;
+        double Rs[NH];
 #pragma unroll
-        for (int tt = 0; tt <= L; ++tt)
-#pragma unroll
-          for (int u = 0; u <= L; ++u)
-#pragma unroll
-            for (int w = 0; w <= L; ++w) Rs[tt][u][w] = 0.0;
-        for (int c = 0; c < natom; ++c) {
+        for (int h = 0; h < NH; ++h) Rs[h] = 0.0;
+        for (int c = c0; c < c1; ++c) {
           const double X = Px - xyz[3 * c], Y = Py - xyz[3 * c + 1], Z = Pz - xyz[3 * c + 2];
           double F[L + 1];
           boys_eval<L>(p * (X * X + Y * Y + Z * Z), boys_lds, F);
-          double R[L + 1][L + 1][L + 1][L + 1];
-          double fac = Zc[c];
+          double fac[L + 1];
+          fac[0] = Zc[c];
 #pragma unroll
-          for (int n = 0; n <= L; ++n) { R[n][0][0][0] = fac * F[n]; fac *= -2.0 * p; }
+          for (int n = 1; n <= L; ++n) fac[n] = fac[n - 1] * (-2.0 * p);
+          double Rc[NH], Rn[NH];
+          Rc[0] = fac[L] * F[L];
 #pragma unroll
-          for (int n = L - 1; n >= 0; --n)
+          for (int n = L - 1; n >= 0; --n) {
+            Rn[0] = fac[n] * F[n];
 #pragma unroll
-            for (int tt = 0; tt <= L; ++tt)
+            for (int sl = 1; sl <= L; ++sl) {
+              if (sl > L - n) break;
 #pragma unroll
-              for (int u = 0; u <= L; ++u)
+              for (int tt = sl; tt >= 0; --tt)
 #pragma unroll
-                for (int w = 0; w <= L; ++w) {
-                  if (tt + u + w == 0 || n + tt + u + w > L) continue;
+                for (int w = 0; w <= sl - tt; ++w) {
+                  const int u = sl - tt - w;
                   double r;
-                  if (tt > 0) r = X * R[n + 1][tt - 1][u][w] + (tt > 1 ? (tt - 1) * R[n + 1][tt - 2][u][w] : 0.0);
-                  else if (u > 0) r = Y * R[n + 1][tt][u - 1][w] + (u > 1 ? (u - 1) * R[n + 1][tt][u - 2][w] : 0.0);
-                  else r = Z * R[n + 1][tt][u][w - 1] + (w > 1 ? (w - 1) * R[n + 1][tt][u][w - 2] : 0.0);
-                  R[n][tt][u][w] = r;
+                  if (tt > 0) r = X * Rc[hidx(tt - 1, u, w)] + (tt > 1 ? (tt - 1) * Rc[hidx(tt - 2, u, w)] : 0.0);
+                  else if (u > 0) r = Y * Rc[hidx(tt, u - 1, w)] + (u > 1 ? (u - 1) * Rc[hidx(tt, u - 2, w)] : 0.0);
+                  else r = Z * Rc[hidx(tt, u, w - 1)] + (w > 1 ? (w - 1) * Rc[hidx(tt, u, w - 2)] : 0.0);
+                  Rn[hidx(tt, u, w)] = r;
                 }
+            }
 #pragma unroll
-          for (int tt = 0; tt <= L; ++tt)
+            for (int h = 0; h < NH; ++h) Rc[h] = Rn[h];
+          }
 #pragma unroll
-            for (int u = 0; u <= L; ++u)
-#pragma unroll
-              for (int w = 0; w <= L; ++w)
-                if (tt + u + w <= L) Rs[tt][u][w] += R[0][tt][u][w];
+          for (int h = 0; h < NH; ++h) Rs[h] += Rc[h];
         }
         const double pref = -2.0 * 3.14159265358979323846 / p * K;
 #pragma unroll
@@ -157,7 +182,7 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
               for (int u = 0; u <= ay + by; ++u) {
                 const double ey = hermE(ay, by, u, PA1, PB1, o2p);
 #pragma unroll
-                for (int w = 0; w <= az + bz; ++w) acc += ex * ey * hermE(az, bz, w, PA2, PB2, o2p) * Rs[tt][u][w];
+                for (int w = 0; w <= az + bz; ++w) acc += ex * ey * hermE(az, bz, w, PA2, PB2, o2p) * Rs[hidx(tt, u, w)];
               }
             }
             s[ia][ib] += pref * acc;   // V accumulates in s in this pass
@@ -170,11 +195,11 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
 #pragma unroll
     for (int ib = 0; ib < NB; ++ib) {
       const long r = pi.a0 + ia, cidx = pi.b0 + ib;
-      M0[r * ncart + cidx] = s[ia][ib];
-      M0[cidx * ncart + r] = s[ia][ib];
+      atomicAdd(&M0[r * ncart + cidx], s[ia][ib]);
+      if (a != b) atomicAdd(&M0[cidx * ncart + r], s[ia][ib]);
       if (!DO_V) {
-        T[r * ncart + cidx] = t[ia][ib];
-        T[cidx * ncart + r] = t[ia][ib];
+        atomicAdd(&T[r * ncart + cidx], t[ia][ib]);
+        if (a != b) atomicAdd(&T[cidx * ncart + r], t[ia][ib]);
       }
     }
 }
@@ -182,17 +207,20 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
 }  // namespace
 
 void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
-                 const double* Z, const double* xyz, int natom, const double* boys, int ncart, double* S, double* T,
-                 double* V, hipStream_t stream) {
+                 int max_prim, const double* Z, const double* xyz, int natom, const double* boys, int ncart,
+                 double* S, double* T, double* V, hipStream_t stream) {
   const int n = p_end - p_begin;
   if (n <= 0) return;
-  const dim3 grid((n + WG - 1) / WG), block(WG);
+  const dim3 grid_st((n + WG - 1) / WG, std::max(1, max_prim));
+  const dim3 grid_v((n + WG - 1) / WG, std::max(1, (natom + ONEE_ATOMS - 1) / ONEE_ATOMS));
+  const dim3 block(WG);
 #define CQ_ONEE(LA, LB)                                                                                 \
   do {                                                                                                   \
-    hipLaunchKernelGGL((onee_kernel<LA, LB, false>), grid, block, 0, stream, pairs, p_begin, p_end, sd, Z, xyz, \
-                       natom, boys, ncart, S, T, V);                                                     \
-    hipLaunchKernelGGL((onee_kernel<LA, LB, true>), grid, block, 0, stream, pairs, p_begin, p_end, sd, Z, xyz,  \
-                       natom, boys, ncart, S, T, V);                                                     \
+    hipLaunchKernelGGL((onee_kernel<LA, LB, false>), grid_st, block, 0, stream, pairs, p_begin, p_end, sd, Z, \
+                       xyz, natom, boys, ncart, S, T, V);                                                \
+    if (natom > 0)                                                                                       \
+      hipLaunchKernelGGL((onee_kernel<LA, LB, true>), grid_v, block, 0, stream, pairs, p_begin, p_end, sd, Z,  \
+                         xyz, natom, boys, ncart, S, T, V);                                              \
   } while (0)
   switch (pair_class) {
     case 0: CQ_ONEE(0, 0); break;

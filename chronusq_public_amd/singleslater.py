@@ -39,6 +39,15 @@ from .molecule import Molecule
 # ground-state multiplicities used for the atomic guesses
 DEFAULT_MULT = {1: 2, 2: 1, 3: 2, 4: 1, 5: 2, 6: 3, 7: 4, 8: 3, 9: 2, 10: 1}
 
+# Converged atomic UHF densities of the SAD guess, per (Z, multiplicity, basis name, device):
+# the same atoms recur across molecules and SCF runs of one process.
+_SAD_ATOMS: Dict[tuple, object] = {}
+
+
+def clear_sad_cache():
+    """Forget the cached atomic densities (the next SAD guess recomputes them)."""
+    _SAD_ATOMS.clear()
+
 
 class SCFNotConverged(RuntimeError):
     """SCF failed to converge within maxSCFIter (singleslater_scf.h:253-255)."""
@@ -55,13 +64,15 @@ class SCFResult:
     P: object = None
     Mz: object = None
     setup_s: float = 0.0
+    setup_parts: Dict[str, float] = field(default_factory=dict)
+    guess_s: float = 0.0
 
 
 class SingleSlater:
     def __init__(self, mol: Molecule, basis, ref: str = "RHF", tau: float = 1e-12, guess: str = "SAD",
                  ndiis: int = 10, damp: float = 0.0, ene_tol: float = 1e-10, den_tol: float = 1e-8,
                  max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None,
-                 incremental: bool = False, rebuild_every: int = 8):
+                 incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True):
         import torch
         self.torch = torch
         self.mol = mol
@@ -74,16 +85,22 @@ class SingleSlater:
         # (the reference carries disabled scaffolding for this, singleslater_scf.h:62-63,
         # singleslater_scfutils.h:423-564)
         self.incremental, self.rebuild_every = incremental, rebuild_every
+        self.sad_cache = sad_cache
         self.ene_tol, self.den_tol, self.max_iter = ene_tol, den_tol, max_iter
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         t0 = time.perf_counter()
         self.ao = aoints or AOIntegrals(self.basis)
         self.ao.thresholdSchwartz = tau
+        t1 = time.perf_counter()
         S, T, V = self.ao.computeAOOneE(mol, self.dev)
         self.S, self.H = S, T + V
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
         w, U = torch.linalg.eigh(S)
         self.X = (U * w.rsqrt()) @ U.T          # S^{-1/2} (Loewdin, aointegrals_builders.cpp:736-769)
         self.Xi = (U * w.sqrt()) @ U.T          # S^{1/2}
+        torch.cuda.synchronize()
+        self.setup_parts = {"context_s": t1 - t0, "one_e_s": t2 - t1, "lowdin_s": time.perf_counter() - t2}
         self.enuc = mol.nuclear_repulsion()
         nel = mol.nelectron
         if self.ref == "RHF":
@@ -140,11 +157,17 @@ class SingleSlater:
         Ps = torch.zeros((self.N, self.N), dtype=torch.float64, device=self.dev)
         cache: Dict[int, object] = {}
         for z in dict.fromkeys(int(v) for v in self.mol.Z):
+            key = (z, DEFAULT_MULT.get(z, 1), self.basis.name, str(self.dev))
+            if self.sad_cache and key in _SAD_ATOMS:
+                cache[z] = _SAD_ATOMS[key]
+                continue
             atom = Molecule(np.array([z]), np.zeros((1, 3)), 0, DEFAULT_MULT.get(z, 1))
             hf = SingleSlater(atom, self.basis.name, ref="UHF", tau=1e-14, guess="CORE", ndiis=self.ndiis,
                               max_iter=512, device=self.dev)
             r = hf.SCF3()
             cache[z] = r.P
+            if self.sad_cache:
+                _SAD_ATOMS[key] = r.P
         for iat, z in enumerate(self.mol.Z):
             idx = np.nonzero(atom_of_bf == iat)[0]
             lo, hi = int(idx[0]), int(idx[-1]) + 1
@@ -166,11 +189,14 @@ class SingleSlater:
             P, Mz = self.sad_guess() if self.guess == "SAD" else self.core_guess()
         else:
             P, Mz = P0, Mz0
+        torch.cuda.synchronize()
+        guess_s = time.perf_counter() - t_start
         uhf = self.ref != "RHF"
         na, nb = self.nocc
         hist_F: List[tuple] = []
         hist_e: List[torch.Tensor] = []
-        res = SCFResult(energy=0.0, iterations=0, converged=False, setup_s=self.setup_s)
+        res = SCFResult(energy=0.0, iterations=0, converged=False, setup_s=self.setup_s, guess_s=guess_s,
+                        setup_parts=dict(self.setup_parts))
         E_old = 0.0
         Fo_prev = None
         P_prev = PT_prev = None
