@@ -11,7 +11,9 @@ Extra fields:
                 time, and time until |E - E_fp64| <= 1e-6 Eh, for the plain FP64 build and the
                 incremental (difference-density) FP64 build with a full rebuild every 8 steps;
                 "_cold_sad" runs recompute the SAD atomic densities, the others take them from
-                the process cache filled by an earlier run.
+                the process cache filled by an earlier run; "mixed_incremental" adds FP32
+                integrals for quartets with Schwarz x Schwarz x max|dD| < 1e-5 inside the
+                incremental build (the full rebuilds stay FP64).
   roofline      FP64 algorithmic FLOPs (McMurchie-Davidson model, counted over the surviving
                 quartets of the dominant class-pair kernel) / that kernel's mean launch time
                 from HIP events recorded on its stream, against the FP64 vector peak.
@@ -143,21 +145,23 @@ def main():
     w = M.water_monomer()   # warm-up: code objects, solver handles
     SingleSlater(w, "cc-pvdz", tau=args.tau).SCF3()
 
-    def run_scf(inc, cold):
+    def run_scf(inc, cold, precision="fp64"):
         if cold:
             clear_sad_cache()   # the SAD guess recomputes its atomic UHF densities
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = SingleSlater(mol, bs, tau=args.tau, guess="SAD", incremental=inc, rebuild_every=8).SCF3()
+        r = SingleSlater(mol, bs, tau=args.tau, guess="SAD", incremental=inc, rebuild_every=8,
+                         precision=precision, mixed_threshold=1e-5).SCF3()
         torch.cuda.synchronize()
         return r, time.perf_counter() - t0
 
     r, wall = run_scf(False, True)
     E_ref, P_conv = r.energy, r.P
     scf = {}
-    for name, inc in (("fp64", False), ("fp64_incremental", True)):
+    for name, inc, prec in (("fp64", False, "fp64"), ("fp64_incremental", True, "fp64"),
+                            ("mixed_incremental", True, "mixed")):
         for cold in (True, False):
-            r, wall = run_scf(inc, cold)
+            r, wall = run_scf(inc, cold, prec)
             key = name + ("_cold_sad" if cold else "")
             scf[key] = {"energy": r.energy, "dE_vs_fp64": r.energy - E_ref, "iterations": r.iterations,
                         "wall_s": wall, "setup_s": r.setup_s, "setup_parts": r.setup_parts,

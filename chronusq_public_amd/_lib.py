@@ -18,6 +18,7 @@ CQ_ERR_DEVICE = -3
 CQ_COULOMB = 0
 CQ_EXCHANGE = 1
 CQ_PREC_FP64 = 0
+CQ_PREC_MIXED = 1
 
 # every symbol include/cqgpu.h declares
 EXPORTED = [

@@ -80,6 +80,15 @@ class AOIntegrals:
     def thresholdSchwartz(self, tau: float):
         check(_lib.load().cq_set_schwarz_threshold(self._h, float(tau)), self._h)
 
+    def setPrecision(self, mode: str = "fp64", threshold: float = 1e-5):
+        """'fp64' (the reference's arithmetic) or 'mixed': quartets whose density-weighted
+        Schwarz bound lies below `threshold` get FP32 integrals (FP64 Boys function and
+        accumulation).  No reference counterpart."""
+        code = {"fp64": _lib.CQ_PREC_FP64, "mixed": _lib.CQ_PREC_MIXED}.get(mode)
+        if code is None:
+            raise ValueError(f"unknown precision mode {mode!r}")
+        check(_lib.load().cq_set_precision_mode(self._h, code, float(threshold)), self._h)
+
     def computeSchwartz(self) -> np.ndarray:
         Q = np.zeros((self.nShell, self.nShell))
         check(_lib.load().cq_get_schwarz(self._h, _dp(Q)), self._h)

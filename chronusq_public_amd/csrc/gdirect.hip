@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     const CPMeta* __restrict__ cps, int ncp, const int* __restrict__ item_start, int nitem,
     const int2* __restrict__ s12, const double* __restrict__ pairQ, const int* __restrict__ grp,
     const int* __restrict__ cls_grp_off, const double* __restrict__ Nmax,
-    const unsigned long long* __restrict__ nglob_bits, int nshell, double tau,
+    const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, double tau_hi,
     const unsigned char* __restrict__ pair_nblk, const unsigned char* __restrict__ pair_nublk,
     uint2* __restrict__ tasks,
     unsigned long long* __restrict__ counts, unsigned long long* __restrict__ qcounts,
@@ -143,7 +143,9 @@ __global__ void __launch_bounds__(WG) screen_kernel(
       const double n34 = Nmax[s3 * nshell + s4], n13 = Nmax[s1 * nshell + s3];
       const double n23 = Nmax[s2 * nshell + s3], n12 = Nmax[s1 * nshell + s2];
       const double mx = fmax(n14, fmax(n24, fmax(n34, fmax(n13, fmax(n23, n12)))));
-      pass = !(__dmul_rn(__dmul_rn(mx, Q12), Q34) < tau);
+      // the reference's test, and the upper end of this launch's bound window
+      const double bound = __dmul_rn(__dmul_rn(mx, Q12), Q34);
+      pass = !(bound < tau) && bound < tau_hi;
     }
     nq += __popcll(__ballot(pass));
     // one task entry per primitive block of the swept pair
@@ -532,6 +534,7 @@ struct cq_ctx {
   std::vector<double> Q;             // nshell x nshell
   double tau = 1e-14;                // include/aointegrals.h:334
   int precision_mode = 0;
+  double mixed_threshold = 1e-5;
   double prim_threshold = 1e-17;
   // class-pair tables: [0] every class pair ket-keyed, [1] with bra-keyed (flipped) class pairs
   std::vector<CPMeta> cps[2];
@@ -981,6 +984,29 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   // bra-keyed segments need the segment kernels for every pass: one density slot, LDS on
   const int tb = (c->lds_mode >= 2 && nslot == 1) ? 1 : 0;
   const std::vector<CPMeta>& cps = c->cps[tb];
+  // mixed precision: a second round screens the bound window [tau, threshold) into the same
+  // task buffers and runs it with FP32 integrals (one density slot on the segment path only)
+  const bool mixed = c->precision_mode == CQ_PREC_MIXED && nslot == 1 && c->lds_mode >= 1 &&
+                     c->mixed_threshold > c->tau;
+  const int nround = mixed ? 2 : 1;
+  for (int round = 0; round < nround; ++round) {
+    const bool fp32 = round == 1;
+    const double lo = mixed && !fp32 ? std::max(c->tau, c->mixed_threshold) : c->tau;
+    const double hi = fp32 ? c->mixed_threshold : INFINITY;
+    if (round > 0) {
+      // the task lists, segments and work queues are rebuilt; quartet counts accumulate
+      ZeroList zr{};
+      auto zero2 = [&](void* p, size_t bytes) {
+        zr.p[zr.count] = (unsigned*)p;
+        zr.n[zr.count] = (long)(bytes / 4);
+        ++zr.count;
+      };
+      zero2(c->d_counts.p, kNumClassPairs * sizeof(unsigned long long));
+      zero2(c->d_nseg.p, kNumClassPairs * sizeof(unsigned));
+      zero2(c->d_work_q.p, kNumClassPairs * sizeof(unsigned));
+      hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(WG), 0, s, zr);
+      HIPCHK(hipGetLastError());
+    }
   // 4. per class pair (on the auxiliary streams, highest angular momentum first: those are the
   //    latency-bound ones): screening into its compacted task list, then ERI + J/K digestion,
   //    two density slots per pass; joined back into s before the finalisation
@@ -1012,7 +1038,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
         const int nuni = m.flip ? m.nbra : m.nket;
         hipLaunchKernelGGL(screen_kernel, dim3(grid_for(nuni, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p,
                            kNumClassPairs, c->d_item_start[tb].p, nuni, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
-                           c->d_cls_grp_off.p, c->d_Nmax.p, c->d_nglob.p, ns, c->tau, c->d_pair_nblk.p,
+                           c->d_cls_grp_off.p, c->d_Nmax.p, c->d_nglob.p, ns, lo, hi, c->d_pair_nblk.p,
                            c->d_pair_nublk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p, c->d_segs.p, c->d_nseg.p, k);
         HIPCHK(hipGetLastError());
       }
@@ -1034,15 +1060,16 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
         // bra-keyed segments (table 1 only flips where the LDS rows fit)
         const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->ncart + (size_t)ck.na * ck.nb) * sizeof(double);
         const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
-        hipLaunchKernelGGL(ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
+        hipLaunchKernelGGL(fp32 ? ck.jksegb32 : ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
                            c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
       } else if (seg_path) {
         const int sgrid = (int)std::min<long long>((long long)persistent_grid() * WG / ck.seg_wg, m.seg_cap * ck.nchunk);
-        hipLaunchKernelGGL(ck.jkseg, dim3(std::max(1, sgrid)), dim3(ck.seg_wg), lds_bytes, sk, tasks,
-                           c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
+        hipLaunchKernelGGL(fp32 ? ck.jkseg32 : ck.jkseg, dim3(std::max(1, sgrid)), dim3(ck.seg_wg), lds_bytes, sk,
+                           tasks, c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
       } else {
+        // (the FP32 round's lists run through the FP64 kernels where the segment path does not fit)
         hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
                            c->d_prims.p, c->d_boys.p, jk);
       }
@@ -1056,6 +1083,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       HIPCHK(hipEventRecord(c->ev_join[a], c->aux[a]));
       HIPCHK(hipStreamWaitEvent(s, c->ev_join[a], 0));
     }
+  }
   }
   // 5. back to the shells' basis and accumulate, every matrix in one launch:
   //    AX += 1/2 scale (G + G'^T); COULOMB G = GJ, G' = (nonsym ? GJ2 : GJ); EXCHANGE with GK/GK2
@@ -1155,9 +1183,10 @@ int cq_get_schwarz_threshold(const cq_ctx_t* c, double* tau) {
 
 int cq_set_precision_mode(cq_ctx_t* c, int mode, double threshold) {
   return guarded(c, [&] {
-    if (mode != CQ_PREC_FP64) throw std::invalid_argument("only CQ_PREC_FP64 is available");
-    (void)threshold;
+    if (mode != CQ_PREC_FP64 && mode != CQ_PREC_MIXED) throw std::invalid_argument("unknown precision mode");
+    if (mode == CQ_PREC_MIXED && !(threshold > 0.0)) throw std::invalid_argument("mixed threshold must be > 0");
     c->precision_mode = mode;
+    if (mode == CQ_PREC_MIXED) c->mixed_threshold = threshold;
   });
 }
 

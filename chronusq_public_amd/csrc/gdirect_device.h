@@ -123,6 +123,15 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ l
   }
 }
 
+// FP32 integrals take the Boys values from the FP64 evaluation, rounded.
+template <int L>
+__device__ __forceinline__ void boys_eval(double T, const double* __restrict__ lds, float* F) {
+  double Fd[L + 1];
+  boys_eval<L>(T, lds, Fd);
+#pragma unroll
+  for (int m = 0; m <= L; ++m) F[m] = (float)Fd[m];
+}
+
 // Stage the Boys slice for total angular momentum L into LDS (whole workgroup).
 __device__ __forceinline__ void stage_boys(const double* __restrict__ tab, int L, double* lds) {
   for (int k = threadIdx.x; k < BOYS_NGRID * BOYS_LDS_W; k += blockDim.x) {

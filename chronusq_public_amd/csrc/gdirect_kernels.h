@@ -123,7 +123,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
   /* Segment kernel (one density slot, WGS threads per workgroup): each workgroup takes     \
      (segment, chunk) items from a queue; the segment's ket is uniform, K columns c/d and J(3,4) accumulate in LDS and are \
      flushed to HBM once per item. */                                                        \
-  template <int WGS>                                                                          \
+  template <int WGS, typename R>                                                              \
   __global__ void __launch_bounds__(WGS) eri_jkseg_##NAME(                                     \
       const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                      \
       const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                     \
@@ -159,7 +159,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
         const PairInfo B = pairs[pair_of(tk.x)];                                              \
         QuartetArgs q;                                                                        \
         make_quartet_uniform_ket(B, tk.x, kk, kblk, active, pairs, prims, q);                 \
-        quartet_jk_##NAME<1, 1>(q, jk, boys_lds, chunk, colK, jcdL);                          \
+        quartet_jk_##NAME<1, 1, R>(q, jk, boys_lds, chunk, colK, jcdL);                       \
       }                                                                                       \
       __syncthreads();                                                                        \
       const PairInfo K = pairs[kk];                                                           \
@@ -206,6 +206,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
 // (NA + NB rows of ncart) and J(1,2) accumulate in LDS and are flushed once per item; J(3,4)
 // goes to HBM per lane.
 #define DEFINE_CLASS_SEGB(NAME, LTOT)                                                           \
+  template <typename R>                                                                         \
   __global__ void __launch_bounds__(WG) eri_jksegb_##NAME(                                      \
       const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                        \
       const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                       \
@@ -241,7 +242,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
         const PairInfo K = pairs[pair_of(tk.y)];                                                \
         QuartetArgs q;                                                                          \
         make_quartet_uniform_bra(K, tk.y, bb, bblk, active, pairs, prims, q);                   \
-        quartet_jk_##NAME<1, 2>(q, jk, boys_lds, chunk, rowK, jabL);                            \
+        quartet_jk_##NAME<1, 2, R>(q, jk, boys_lds, chunk, rowK, jabL);                         \
       }                                                                                         \
       __syncthreads();                                                                          \
       const PairInfo A = pairs[bb];                                                             \
@@ -282,6 +283,7 @@ struct ClassKernels {
   SegKernel jksegb;   // bra-uniform segment kernel, or null
   int nc, nd, na, nb;
   int seg_wg;         // workgroup size of jkseg
+  SegKernel jkseg32, jksegb32;   // the same with FP32 integrals (mixed precision)
 };
 
 

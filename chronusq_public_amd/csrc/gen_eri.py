@@ -51,6 +51,15 @@ def hermite_list(L):
     return out
 
 
+# Arithmetic type of the generated quartet code: the template parameter R (double, or float
+# for the mixed-precision kernels).  Literals are wrapped so that they do not promote R.
+RT = "R"
+
+
+def lit(v: float) -> str:
+    return f"R({float(v)})"
+
+
 class Emitter:
     def __init__(self):
         self.lines: List[str] = []
@@ -90,7 +99,7 @@ def gen_E(em: Emitter, pre: str, d: str, LA: int, LB: int, PA: str, PB: str, o2p
                 if (si, sj, t) in E:
                     parts.append(term(X, E[(si, sj, t)]))
                 if (si, sj, t + 1) in E:
-                    parts.append(term(f"{float(t + 1)}", E[(si, sj, t + 1)]))
+                    parts.append(term(lit(t + 1), E[(si, sj, t + 1)]))
                 parts = [p for p in parts if p is not None]
                 if not parts:
                     continue
@@ -100,7 +109,7 @@ def gen_E(em: Emitter, pre: str, d: str, LA: int, LB: int, PA: str, PB: str, o2p
                 if len(parts) == 1 and ("*" not in parts[0]) and ("+" not in parts[0]):
                     E[(i, j, t)] = parts[0]
                 else:
-                    em(f"const double {name} = {expr};")
+                    em(f"const {RT} {name} = {expr};")
                     E[(i, j, t)] = name
     if sign:
         out = {}
@@ -151,11 +160,10 @@ class RGen:
             X, k = "PQz", v - 1
         name = f"R{n}_{t}_{u}_{v}"
         if b is not None:
-            kk = f"{float(k)}"
-            expr = f"{X}*{a} + {b}" if k == 1 else f"{X}*{a} + {kk}*{b}"
+            expr = f"{X}*{a} + {b}" if k == 1 else f"{X}*{a} + {lit(k)}*{b}"
         else:
             expr = f"{X}*{a}"
-        self.em(f"const double {name} = {expr};")
+        self.em(f"const {RT} {name} = {expr};")
         self.memo[key] = name
         return name
 
@@ -209,21 +217,24 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
         # (c, d) accumulate in LDS (colK) and J(3,4) is reduced across the wavefront into LDS (jcdL);
         # 2 = the bra is wavefront-uniform: K rows (a, b) accumulate in LDS (colK holds the rows)
         # and J(1,2) is reduced into LDS (jcdL holds J(a,b)), J(3,4) is a per-lane HBM atomic.
-        em("template <int NSLOT, int DIG>")
+        # R: arithmetic type of the integrals and their digestion (LDS and HBM sums stay double)
+        em("template <int NSLOT, int DIG, typename R = double>")
         em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds, int chunk, double* colK, double* jcdL) {{")
     else:
         em(f"__device__ __forceinline__ void quartet_block_{name}(const QuartetArgs& q, double* __restrict__ out, const double* __restrict__ boys_lds) {{")
     em.ind = 1
+    if kind != "jk":
+        em("using R = double;")
     if kind == "jk" and persist:
         for ia in range(NA):
             for ib in range(NB):
-                em(f"double jab_{ia}_{ib}[NSLOT] = {{}};")
+                em(f"R jab_{ia}_{ib}[NSLOT] = {{}};")
         for ia in range(NA):
             for idd in range(ND):
-                em(f"double kad_{ia}_{idd}[NSLOT] = {{}};")
+                em(f"R kad_{ia}_{idd}[NSLOT] = {{}};")
         for ib in range(NB):
             for idd in range(ND):
-                em(f"double kbd_{ib}_{idd}[NSLOT] = {{}};")
+                em(f"R kbd_{ib}_{idd}[NSLOT] = {{}};")
     for ci, ks in enumerate(chunks):
         if kind == "jk" and not persist:
             # chunks without shared accumulators are independent: one per wavefront
@@ -234,37 +245,42 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
         for ia in range(NA):
             for ib in range(NB):
                 for (ic, idd) in ks:
-                    em(f"double o_{ia}_{ib}_{ic}_{idd} = 0.0;")
+                    em(f"R o_{ia}_{ib}_{ic}_{idd} = R(0);")
         em("for (int ip = q.bra_p0; ip < q.bra_p1; ++ip) {")
         em.ind += 1
         em("const PrimPair bp = q.prim[ip];")
+        em("const R bp_K = R(bp.K), bp_o2p = R(bp.o2p);")
+        em("const R bp_PA0 = R(bp.PA[0]), bp_PA1 = R(bp.PA[1]), bp_PA2 = R(bp.PA[2]);")
+        em("const R bp_PB0 = R(bp.PB[0]), bp_PB1 = R(bp.PB[1]), bp_PB2 = R(bp.PB[2]);")
         Ebra = {}
         for d, i in (("x", 0), ("y", 1), ("z", 2)):
-            Ebra[d] = gen_E(em, "Eb", d, LA, LB, f"bp.PA[{i}]", f"bp.PB[{i}]", "bp.o2p", sign=False)
+            Ebra[d] = gen_E(em, "Eb", d, LA, LB, f"bp_PA{i}", f"bp_PB{i}", "bp_o2p", sign=False)
         for (t, u, v) in herm_ab:
             for (ic, idd) in ks:
-                em(f"double W_{t}_{u}_{v}_{ic}_{idd} = 0.0;")
+                em(f"R W_{t}_{u}_{v}_{ic}_{idd} = R(0);")
         em("for (int iq = q.ket_p0; iq < q.ket_p1; ++iq) {")
         em.ind += 1
         em("const PrimPair kp = q.prim[iq];")
-        em("const double pq = bp.p + kp.p;")
-        em("const double rs = rsqrt_f64(pq);")
-        em("const double rr = rs * rs;")
-        em("const double alpha = bp.p * kp.p * rr;")
-        em("const double PQx = bp.P[0] - kp.P[0];")
-        em("const double PQy = bp.P[1] - kp.P[1];")
-        em("const double PQz = bp.P[2] - kp.P[2];")
-        em("const double Targ = alpha * (PQx * PQx + PQy * PQy + PQz * PQz);")
-        em(f"double F[{L + 1}];")
-        em(f"boys_eval<{L}>(Targ, boys_lds, F);")
-        em("double pw = TWO_PI_52 * kp.K * rs;")
+        # the Boys argument and PQ are formed in double, the rest in R
+        em("const double pqd = bp.p + kp.p;")
+        em("const double rsd = rsqrt_f64(pqd);")
+        em("const double alphad = bp.p * kp.p * (rsd * rsd);")
+        em("const double PQxd = bp.P[0] - kp.P[0], PQyd = bp.P[1] - kp.P[1], PQzd = bp.P[2] - kp.P[2];")
+        em(f"R F[{L + 1}];")
+        em(f"boys_eval<{L}>(alphad * (PQxd * PQxd + PQyd * PQyd + PQzd * PQzd), boys_lds, F);")
+        em("const R PQx = R(PQxd), PQy = R(PQyd), PQz = R(PQzd);")
+        em("const R alpha = R(alphad);")
+        em("R pw = R(TWO_PI_52 * kp.K * rsd);")
         for n in range(L + 1):
-            em(f"const double Rb{n} = F[{n}] * pw;")
+            em(f"const R Rb{n} = F[{n}] * pw;")
             if n < L:
-                em("pw *= -2.0 * alpha;")
+                em("pw *= R(-2.0) * alpha;")
+        em("const R kp_o2p = R(kp.o2p);")
+        em("const R kp_PA0 = R(kp.PA[0]), kp_PA1 = R(kp.PA[1]), kp_PA2 = R(kp.PA[2]);")
+        em("const R kp_PB0 = R(kp.PB[0]), kp_PB1 = R(kp.PB[1]), kp_PB2 = R(kp.PB[2]);")
         Eket = {}
         for d, i in (("x", 0), ("y", 1), ("z", 2)):
-            Eket[d] = gen_E(em, "Ek", d, LC, LD, f"kp.PA[{i}]", f"kp.PB[{i}]", "kp.o2p", sign=True)
+            Eket[d] = gen_E(em, "Ek", d, LC, LD, f"kp_PA{i}", f"kp_PB{i}", "kp_o2p", sign=True)
         rg = RGen(em, L)
         for (ic, idd) in ks:
             (kx, ky, kz), (lx, ly, lz) = cc[ic], cd[idd]
@@ -282,7 +298,7 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                         cname = None
                         if e is not None and "*" in e:
                             cname = f"C_{ic}_{idd}_{tau}_{nu}_{ph}"
-                            em(f"const double {cname} = {'-' if sg < 0 else ''}{e};")
+                            em(f"const R {cname} = {'-' if sg < 0 else ''}{e};")
                             sg = 1
                         elif e is not None:
                             cname = e
@@ -309,7 +325,7 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                                 continue
                             e, _ = mul(Ebra["x"][(ax, bx, t)], Ebra["y"][(ay, by, u)], Ebra["z"][(az, bz, v)])
                             bn = f"B_{ia}_{ib}_{t}_{u}_{v}"
-                            em(f"const double {bn} = bp.K{('*' + e) if e else ''};")
+                            em(f"const R {bn} = bp_K{('*' + e) if e else ''};")
                             bterms.append((t, u, v, bn))
                 for (ic, idd) in ks:
                     expr = " + ".join(f"{bn}*W_{t}_{u}_{v}_{ic}_{idd}" for (t, u, v, bn) in bterms)
@@ -381,8 +397,8 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
     em.ind += 1
     em("const double* __restrict__ D = jk.D[s];")
     _jk_addressing(em)
-    em("const double dg = q.deg;")
-    em("const double hk = 0.5 * q.deg;")
+    em("const R dg = R(q.deg);")
+    em("const R hk = R(0.5 * q.deg);")
     em("const bool ns = jk.nonsym[s] != 0;")
     em("const bool live = q.deg != 0.0;   // lanes without a quartet in this pass skip their atomics")
     em("if (jk.do_j[s]) {")
@@ -390,35 +406,35 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
     if not persist:
         for ia in range(NA):
             for ib in range(NB):
-                em(f"double jab_{ia}_{ib} = 0.0;")
+                em(f"R jab_{ia}_{ib} = R(0);")
     for (ic, idd) in ks:
-        em(f"{{ const double x43 = D[{_cd('xdc', ic, idd)}];")
-        em("  double jcd = 0.0;")
+        em(f"{{ const R x43 = R(D[{_cd('xdc', ic, idd)}]);")
+        em("  R jcd = R(0);")
         for ia in range(NA):
             for ib in range(NB):
-                em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('xba', ia, ib)}];")
+                em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * R(D[{_ab('xba', ia, ib)}]);")
                 tgt = f"jab_{ia}_{ib}[s]" if persist else f"jab_{ia}_{ib}"
                 em(f"  {tgt} += o_{ia}_{ib}_{ic}_{idd} * x43;")
-        em(f"  if constexpr (DIG == 1) wave_add_lds(&jcdL[s * {NC * ND} + {ic * ND + idd}], dg * jcd);")
-        em(f"  else if constexpr (DIG == 2) {{ if (live) atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd); }}")
-        em(f"  else wave_add(&jk.GJ[s][{_cd('tcd', ic, idd)}], dg * jcd, q.ket_uniform); }}")
+        em(f"  if constexpr (DIG == 1) wave_add_lds(&jcdL[s * {NC * ND} + {ic * ND + idd}], double(dg * jcd));")
+        em(f"  else if constexpr (DIG == 2) {{ if (live) atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], double(dg * jcd)); }}")
+        em(f"  else wave_add(&jk.GJ[s][{_cd('tcd', ic, idd)}], double(dg * jcd), q.ket_uniform); }}")
     if not persist:
         for ia in range(NA):
             for ib in range(NB):
-                em(f"if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], dg * jab_{ia}_{ib});")
-                em(f"else if (live) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib});")
+                em(f"if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], double(dg * jab_{ia}_{ib}));")
+                em(f"else if (live) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], double(dg * jab_{ia}_{ib}));")
     em("if (ns) {")
     em.ind += 1
     for (ic, idd) in ks:
-        em(f"{{ const double x34 = dg * D[{_cd('tcd', ic, idd)}];")
-        em("  double jcd = 0.0;")
+        em(f"{{ const R x34 = dg * R(D[{_cd('tcd', ic, idd)}]);")
+        em("  R jcd = R(0);")
         for ia in range(NA):
             for ib in range(NB):
-                em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * D[{_ab('tab', ia, ib)}];")
+                em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * R(D[{_ab('tab', ia, ib)}]);")
         for ia in range(NA):
             for ib in range(NB):
-                em(f"  if (live) atomicAdd(&jk.GJ2[s][{_ab('tab', ia, ib)}], o_{ia}_{ib}_{ic}_{idd} * x34);")
-        em(f"  wave_add(&jk.GJ2[s][{_cd('tcd', ic, idd)}], dg * jcd, q.ket_uniform); }}")
+                em(f"  if (live) atomicAdd(&jk.GJ2[s][{_ab('tab', ia, ib)}], double(o_{ia}_{ib}_{ic}_{idd} * x34));")
+        em(f"  wave_add(&jk.GJ2[s][{_cd('tcd', ic, idd)}], double(dg * jcd), q.ket_uniform); }}")
     em.ind -= 1
     em("}")
     em.ind -= 1
@@ -428,21 +444,21 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
 
     def X(r0, ri, c0, ci, trans):
         if trans:
-            return f"D[(q.{c0} + {ci}) * N + q.{r0} + {ri}]"
-        return f"D[(q.{r0} + {ri}) * N + q.{c0} + {ci}]"
+            return f"R(D[(q.{c0} + {ci}) * N + q.{r0} + {ri}])"
+        return f"R(D[(q.{r0} + {ri}) * N + q.{c0} + {ci}])"
 
     NCOL = NC + ND
 
     def kadd(G, trans, row0, ri, colkind, ci, val):
         """One K update: row (row0 + ri), column c0 + ci (colkind 'c') or d0 + ci ('d')."""
         col0 = "q.c0" if colkind == "c" else "q.d0"
-        glob = f"atomicAdd(&{G}[(q.{row0} + {ri}) * N + {col0} + {ci}], {val})"
+        glob = f"atomicAdd(&{G}[(q.{row0} + {ri}) * N + {col0} + {ci}], double({val}))"
         if trans:
             return f"if (live) {glob};"
         lcol = ci if colkind == "c" else NC + ci
-        lds = f"atomicAdd(&colK[(s * {NCOL} + {lcol}) * N + q.{row0} + {ri}], {val})"
+        lds = f"atomicAdd(&colK[(s * {NCOL} + {lcol}) * N + q.{row0} + {ri}], double({val}))"
         lrow = ri if row0 == "a0" else NA + ri
-        ldsb = f"atomicAdd(&colK[(s * {NA + NB} + {lrow}) * N + {col0} + {ci}], {val})"
+        ldsb = f"atomicAdd(&colK[(s * {NA + NB} + {lrow}) * N + {col0} + {ci}], double({val}))"
         return (f"if (live) {{ if constexpr (DIG == 1) {lds}; else if constexpr (DIG == 2) {ldsb}; "
                 f"else {glob}; }}")
 
@@ -450,7 +466,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
         # K(a,c) += X(b,d) ; K(b,c) += X(a,d)   (partial over the chunk's d)
         for ia in range(NA):
             for ic in cs:
-                em("{ double acc = 0.0;")
+                em("{ R acc = R(0);")
                 for ib in range(NB):
                     for (jc, idd) in ks:
                         if jc == ic:
@@ -458,7 +474,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
                 em(f"  {kadd(G, trans, 'a0', ia, 'c', ic, 'hk * acc')} }}")
         for ib in range(NB):
             for ic in cs:
-                em("{ double acc = 0.0;")
+                em("{ R acc = R(0);")
                 for ia in range(NA):
                     for (jc, idd) in ks:
                         if jc == ic:
@@ -469,7 +485,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
         # K(a,d) += X(b,c) ; K(b,d) += X(a,c)   (partial over the chunk's c)
         for ia in range(NA):
             for idd in ds:
-                em("{ double acc = 0.0;")
+                em("{ R acc = R(0);")
                 for ib in range(NB):
                     for (ic, jd) in ks:
                         if jd == idd:
@@ -477,7 +493,7 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
                 em(f"  {kadd(G, trans, 'a0', ia, 'd', idd, 'hk * acc')} }}")
         for ib in range(NB):
             for idd in ds:
-                em("{ double acc = 0.0;")
+                em("{ R acc = R(0);")
                 for ia in range(NA):
                     for (ic, jd) in ks:
                         if jd == idd:
@@ -488,14 +504,14 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
     if persist:
         for ic in cs:
             for ib in range(NB):
-                em(f"{{ const double xbc = {X('b0', ib, 'c0', ic, False)};")
+                em(f"{{ const R xbc = {X('b0', ib, 'c0', ic, False)};")
                 for ia in range(NA):
                     for (jc, idd) in ks:
                         if jc == ic:
                             em(f"  kad_{ia}_{idd}[s] += o_{ia}_{ib}_{ic}_{idd} * xbc;")
                 em("}")
             for ia in range(NA):
-                em(f"{{ const double xac = {X('a0', ia, 'c0', ic, False)};")
+                em(f"{{ const R xac = {X('a0', ia, 'c0', ic, False)};")
                 for ib in range(NB):
                     for (jc, idd) in ks:
                         if jc == ic:
@@ -520,25 +536,25 @@ def gen_digest_final(em: Emitter, NA, NB, NC, ND):
     em("const int N = jk.ncart;")
     em("const int tab = q.bra_swap ? q.b0 * N + q.a0 : q.a0 * N + q.b0;")
     em("const int tab_a = q.bra_swap ? 1 : N, tab_b = q.bra_swap ? N : 1;")
-    em("const double dg = q.deg;")
-    em("const double hk = 0.5 * q.deg;")
+    em("const R dg = R(q.deg);")
+    em("const R hk = R(0.5 * q.deg);")
     em("if (jk.do_j[s]) {")
     for ia in range(NA):
         for ib in range(NB):
-            em(f"  if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], dg * jab_{ia}_{ib}[s]);")
-            em(f"  else if (q.deg != 0.0) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], dg * jab_{ia}_{ib}[s]);")
+            em(f"  if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], double(dg * jab_{ia}_{ib}[s]));")
+            em(f"  else if (q.deg != 0.0) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], double(dg * jab_{ia}_{ib}[s]));")
     em("}")
     em("if (q.deg != 0.0 && jk.do_k[s]) {")
     for ia in range(NA):
         for idd in range(ND):
-            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.a0 + {ia}], hk * kad_{ia}_{idd}[s]);")
-            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
-            em(f"  else atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], hk * kad_{ia}_{idd}[s]);")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.a0 + {ia}], double(hk * kad_{ia}_{idd}[s]));")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {ia}) * N + q.d0 + {idd}], double(hk * kad_{ia}_{idd}[s]));")
+            em(f"  else atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], double(hk * kad_{ia}_{idd}[s]));")
     for ib in range(NB):
         for idd in range(ND):
-            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.b0 + {ib}], hk * kbd_{ib}_{idd}[s]);")
-            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {NA + ib}) * N + q.d0 + {idd}], hk * kbd_{ib}_{idd}[s]);")
-            em(f"  else atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], hk * kbd_{ib}_{idd}[s]);")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * N + q.b0 + {ib}], double(hk * kbd_{ib}_{idd}[s]));")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {NA + ib}) * N + q.d0 + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
+            em(f"  else atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
     em("}")
     em.ind -= 1
     em("}")
@@ -591,9 +607,10 @@ def main(out_dir: str, maxl: int = 2):
             seg_wg = 512 if (LA, LB) in ((0, 0), (1, 0)) else 256
             src.append(f"void cq_register_{name}(ClassKernels& ck) {{")
             src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, kChunks_{name}, \"{name}\", "
-                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}<{seg_wg}>, "
-                       f"{('eri_jksegb_' + name) if segb else 'nullptr'}, kNC_{name}, kND_{name}, kNA_{name}, kNB_{name}, "
-                       f"{seg_wg}}};")
+                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}<{seg_wg}, double>, "
+                       f"{('eri_jksegb_' + name + '<double>') if segb else 'nullptr'}, kNC_{name}, kND_{name}, "
+                       f"kNA_{name}, kNB_{name}, {seg_wg}, eri_jkseg_{name}<{seg_wg}, float>, "
+                       f"{('eri_jksegb_' + name + '<float>') if segb else 'nullptr'}}};")
             src.append("}")
             _write_if_changed(os.path.join(out_dir, f"eri_{name}.hip"), "\n".join(src) + "\n")
             table.append(name)

@@ -72,7 +72,8 @@ class SingleSlater:
     def __init__(self, mol: Molecule, basis, ref: str = "RHF", tau: float = 1e-12, guess: str = "SAD",
                  ndiis: int = 10, damp: float = 0.0, ene_tol: float = 1e-10, den_tol: float = 1e-8,
                  max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None,
-                 incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True):
+                 incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True,
+                 precision: str = "fp64", mixed_threshold: float = 1e-5):
         import torch
         self.torch = torch
         self.mol = mol
@@ -91,6 +92,9 @@ class SingleSlater:
         t0 = time.perf_counter()
         self.ao = aoints or AOIntegrals(self.basis)
         self.ao.thresholdSchwartz = tau
+        # mixed: FP32 integrals for quartets whose bound (inside the incremental build:
+        # Schwarz x Schwarz x max|dD|) is below mixed_threshold; full FP64 rebuilds stay FP64
+        self.precision, self.mixed_threshold = precision, mixed_threshold
         t1 = time.perf_counter()
         S, T, V = self.ao.computeAOOneE(mol, self.dev)
         self.S, self.H = S, T + V
@@ -198,6 +202,7 @@ class SingleSlater:
         res = SCFResult(energy=0.0, iterations=0, converged=False, setup_s=self.setup_s, guess_s=guess_s,
                         setup_parts=dict(self.setup_parts))
         E_old = 0.0
+        last_dE = float("inf")
         Fo_prev = None
         P_prev = PT_prev = None
         B_hist = None
@@ -206,7 +211,12 @@ class SingleSlater:
             torch.cuda.synchronize()
             tg = time.perf_counter()
             if self.incremental and not uhf and it % self.rebuild_every != 0:
+                # FP32 only while the energy still moves by more than 1e-7 Eh per step: near
+                # convergence its rounding would hold the 1e-10 / 1e-8 criteria back
+                if self.precision == "mixed" and abs(last_dE) > 1e-7:
+                    self.ao.setPrecision("mixed", self.mixed_threshold)
                 dPT, _ = self.formPT(P - P_prev)
+                self.ao.setPrecision("fp64")
                 PT = PT_prev + dPT
                 PTz = None
             else:
@@ -271,6 +281,7 @@ class SingleSlater:
             if uhf:
                 dP = max(dP, float(torch.linalg.norm(Mzn - Mz)))
             dE = E - E_old
+            last_dE = dE
             E_old = E
             if abs(dE) < 1e-6:
                 damp = 0.0

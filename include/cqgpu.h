@@ -43,6 +43,7 @@ typedef struct cq_ctx cq_ctx_t;
 
 /* precision modes */
 #define CQ_PREC_FP64 0
+#define CQ_PREC_MIXED 1
 
 /* Create a context from a flat shell list (replaces the BasisSet/AOIntegrals state that
  * newTwoEContractDirect reads, aointegrals_contract.h:45-48,96-109).
@@ -64,7 +65,14 @@ const char* cq_last_error(const cq_ctx_t* ctx);
 int cq_set_schwarz_threshold(cq_ctx_t* ctx, double tau);
 int cq_get_schwarz_threshold(const cq_ctx_t* ctx, double* tau);
 
-/* Precision mode of the quartet arithmetic; only CQ_PREC_FP64 is provided. */
+/* Precision of the quartet arithmetic (no reference counterpart: the reference is FP64).
+ * CQ_PREC_FP64: every quartet in FP64.  CQ_PREC_MIXED: quartets whose density-weighted
+ * Schwarz bound (the screening bound max|X block| Q12 Q34) lies in [tau, threshold) are
+ * evaluated with FP32 integrals (Boys function, J/K accumulation into G and the reduction
+ * stay FP64); bounds >= threshold stay FP64.  Inside an incremental Fock build X is the
+ * density difference, so the bound is Schwarz(ab) Schwarz(cd) max|dD|.  The FP32 kernels
+ * serve one-density contractions on the LDS segment path (RHF G[D]); other calls run FP64
+ * throughout.  threshold > 0 (1e-5 by default). */
 int cq_set_precision_mode(cq_ctx_t* ctx, int mode, double threshold);
 
 /* Sizes: shells, basis functions, Cartesian functions, shell pairs, primitive pairs kept. */

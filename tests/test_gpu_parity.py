@@ -210,3 +210,39 @@ def test_every_digestion_mode_matches_oracle(lds):
     (Jo, Ko), n = _oracle_jk(ob, Qg, [X, X], [0, 1], [1.0, -0.5], 1e-12)
     assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL
     assert sum(ao.quartet_counts().values()) == n
+
+
+def test_mixed_precision_window_matches_fp64():
+    """Mixed mode: quartets whose bound lies in [tau, 1e-5) get FP32 integrals.  On a
+    difference-density-sized X ((H2O)4 / cc-pVDZ, |X| ~ 1e-3) most quartets fall in the FP32
+    window; the screening decisions are unchanged (same surviving-quartet count) and G agrees
+    with the FP64 result within the FP32 rounding of the window's contributions (each below
+    1e-5, relative error ~1e-7): bound 1e-9 + 1e-6 max|G|."""
+    mol = M.water_cluster(4, seed=3, box_ang=M.scaled_box(4))
+    bs, ao, ob = _setup(mol, "cc-pvdz")
+    X = np.asfortranarray(_sym_random(bs.nbasis, 31) * 1e-3)
+    J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+    n64 = sum(ao.quartet_counts().values())
+    ao.setPrecision("mixed", 1e-5)
+    try:
+        Jm, Km = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+        nmix = sum(ao.quartet_counts().values())
+    finally:
+        ao.setPrecision("fp64")
+    assert nmix == n64
+    for a, b in ((J, Jm), (K, Km)):
+        err = np.abs(a - b).max()
+        assert err < 1e-9 + 1e-6 * np.abs(a).max(), err
+        assert err > 0.0   # the FP32 window was exercised
+    # FP64 again after switching back: bitwise the first result
+    J2, K2 = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+    assert np.abs(J2 - J).max() < 1e-13 and np.abs(K2 - K).max() < 1e-13
+
+
+def test_precision_mode_errors():
+    from chronusq_public_amd._lib import CQError
+    bs, ao, ob = _setup(M.water_test0001(), "6-31g*")
+    with pytest.raises(ValueError):
+        ao.setPrecision("half")
+    with pytest.raises(CQError):
+        ao.setPrecision("mixed", 0.0)

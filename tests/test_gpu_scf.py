@@ -90,3 +90,16 @@ def test_incremental_fock_reaches_fp64_energy():
     inc = SingleSlater(mol, "cc-pvdz", tau=1e-12, incremental=True, rebuild_every=4).SCF3()
     assert plain.converged and inc.converged
     assert abs(inc.energy - plain.energy) < 1e-9
+
+
+def test_mixed_precision_incremental_scf_energy():
+    """Mixed precision inside the incremental build (FP32 where Schwarz x Schwarz x max|dD| <
+    1e-5, full FP64 rebuild every 4 iterations) converges to the FP64 energy well within the
+    1e-6 Eh goal ((H2O)3 / cc-pVDZ): |dE| < 1e-8 Eh."""
+    from chronusq_public_amd.singleslater import SingleSlater
+    mol = M.water_cluster(3, seed=2, box_ang=M.scaled_box(3))
+    plain = SingleSlater(mol, "cc-pvdz", tau=1e-12).SCF3()
+    mixed = SingleSlater(mol, "cc-pvdz", tau=1e-12, incremental=True, rebuild_every=4,
+                         precision="mixed", mixed_threshold=1e-5).SCF3()
+    assert mixed.converged
+    assert abs(mixed.energy - plain.energy) < 1e-8
