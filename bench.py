@@ -36,6 +36,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of each class-pair kernel (FP64 integrals) on this workload: rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors taken from a 256 MiB device copy measured
+# in the same runs (profiles/gdirect_h2o16_ccpvdz_counters.md).  Most of the writes are the
+# FP64 atomics of the J/K digestion.
+TRAFFIC_BYTES = {
+    "(dd|dd)": 28332524, "(dp|dp)": 196870144, "(dp|ds)": 90397737, "(ds|ds)": 34980495,
+    "(pp|ds)": 108538102, "(pp|pp)": 62159585, "(ps|ds)": 189792256, "(ps|ps)": 359451853,
+    "(ss|dp)": 56172216, "(ss|ds)": 103064535, "(ss|pp)": 117230797, "(ss|ps)": 375830610,
+    "(ss|ss)": 156478996, "(dd|dp)": 27294433, "(dd|ds)": 12184453, "(dd|pp)": 18979226,
+    "(dd|ps)": 21390950, "(dd|ss)": 10999562, "(dp|pp)": 91152712, "(dp|ps)": 68398612,
+    "(pp|ps)": 164626432,
+}
 METRIC = "SCF time-to-energy (1e-6 Eh vs FP64) + G[D] quartets/s, (H2O)16 cc-pVDZ"
 
 
@@ -256,7 +268,7 @@ def main():
                    "step_tflops": total_flops / (ms_per_step * 1e-3) / 1e12},
         "roofline": {"bound": "mfma", "pipe": "FP64 VALU (MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s)", "kernel": "eri_jk" + dom["class"], "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None},
+                     "traffic": TRAFFIC_BYTES.get(dom["class"])},
         "scf": scf,
         "classes": sorted(per_class, key=lambda r: -r["kernel_ms"]),
         "wall_s_timed": wall,
