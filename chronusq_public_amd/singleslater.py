@@ -20,6 +20,11 @@ same device.  Steps and conventions follow the reference:
   |dE| > 1e-6 (off by default here: it only lengthens the path to the same energy).
 * convergence (singleslater_scfutils.h:262-355): |dE| < eneTol and ||dP||_F < denTol
   (1e-10, 1e-8, include/singleslater.h:448-449); non-convergence raises.
+* density step: the reference diagonalises the orthonormal Fock every iteration
+  (singleslater_scfutils.h:567-581).  Here RHF does so while |dE| > 0.1 Eh and then rotates
+  the previous orbitals to first order in F_ov (pseudo-diagonalisation, GEMMs only: the
+  eigensolve is launch-bound on the GPU); the fixed point F_ov = 0 is the same, and so is the
+  converged energy.
 * SAD guess (singleslater_guess.h:99-222): one UHF per unique element at its default
   multiplicity, atomic scalar densities on the diagonal blocks, spin-averaged, one Fock build
   and diagonalisation.  The atomic UHFs use CDIIS (the reference switches DIIS off for them).
@@ -73,7 +78,8 @@ class SingleSlater:
                  ndiis: int = 10, damp: float = 0.0, ene_tol: float = 1e-10, den_tol: float = 1e-8,
                  max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None,
                  incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True,
-                 precision: str = "fp64", mixed_threshold: float = 1e-5):
+                 precision: str = "fp64", mixed_threshold: float = 1e-5,
+                 pseudo_diag: bool = True, pdiag_switch: float = 1e-1):
         import torch
         self.torch = torch
         self.mol = mol
@@ -95,6 +101,9 @@ class SingleSlater:
         # mixed: FP32 integrals for quartets whose bound (inside the incremental build:
         # Schwarz x Schwarz x max|dD|) is below mixed_threshold; full FP64 rebuilds stay FP64
         self.precision, self.mixed_threshold = precision, mixed_threshold
+        # RHF density step: the eigensolve while |dE| > pdiag_switch, then first-order
+        # orbital rotations (pseudo-diagonalisation, GEMMs only)
+        self.pseudo_diag, self.pdiag_switch = pseudo_diag, pdiag_switch
         t1 = time.perf_counter()
         S, T, V = self.ao.computeAOOneE(mol, self.dev)
         self.S, self.H = S, T + V
@@ -130,12 +139,37 @@ class SingleSlater:
         self.ao.newTwoEContractDirect([P, P, Mz], [J, K, Kz], [COULOMB, EXCHANGE, EXCHANGE], [1.0, -0.5, -0.5])
         return J + K, Kz
 
-    def _diag_density(self, F, nocc):
+    def _diag_density(self, F, nocc, with_vectors=False):
         torch = self.torch
         Fo = self.X @ F @ self.X
         e, C = torch.linalg.eigh(Fo)
         Cao = self.X @ C[:, :nocc]
+        if with_vectors:
+            return Cao @ Cao.T, C
         return Cao @ Cao.T, e
+
+    def _pseudo_diag_density(self, F, nocc, C):
+        """Density from a first-order occupied-virtual rotation of the previous orbitals
+        (orthonormal basis): kappa_ai = F_ai / (e_i - e_a) with e the diagonal of F in the
+        previous orbitals, C_occ += C_vir kappa, C_vir -= C_occ kappa^T, then Newton-Schulz
+        re-orthonormalisation C <- C (3 I - C^T C) / 2.  GEMMs only, in place of the
+        eigensolve once the SCF is near its fixed point (F_ov = 0 there, so the converged
+        density is the aufbau density the eigensolve gives).  Returns None when an
+        occupied-virtual gap is below 1e-3 Eh (near-degenerate frontier orbitals)."""
+        torch = self.torch
+        Fo = self.X @ F @ self.X
+        FC = Fo @ C
+        eps = (C * FC).sum(dim=0)
+        eo, ev = eps[:nocc], eps[nocc:]
+        den = eo[None, :] - ev[:, None]
+        if float(den.abs().min()) < 1e-3:
+            return None
+        kap = (C[:, nocc:].T @ FC[:, :nocc]) / den
+        Cn = torch.cat([C[:, :nocc] + C[:, nocc:] @ kap, C[:, nocc:] - C[:, :nocc] @ kap.T], dim=1)
+        for _ in range(3):
+            Cn = 1.5 * Cn - 0.5 * Cn @ (Cn.T @ Cn)
+        Cao = self.X @ Cn[:, :nocc]
+        return Cao @ Cao.T, Cn
 
     def _energy(self, P, PT, Mz=None, PTz=None):
         E = float((P * self.H).sum() + 0.25 * (P * PT).sum()) + self.enuc
@@ -203,6 +237,7 @@ class SingleSlater:
                         setup_parts=dict(self.setup_parts))
         E_old = 0.0
         last_dE = float("inf")
+        C_orb = None    # orthonormal-basis orbitals of the last density step (RHF)
         Fo_prev = None
         P_prev = PT_prev = None
         B_hist = None
@@ -275,7 +310,12 @@ class SingleSlater:
                 Pb_n, _ = self._diag_density(Fs[1], nb)
                 Pn, Mzn = Pa_n + Pb_n, Pa_n - Pb_n
             else:
-                Pa_n, _ = self._diag_density(Fs[0], na)
+                out = None
+                if self.pseudo_diag and C_orb is not None and abs(last_dE) < self.pdiag_switch:
+                    out = self._pseudo_diag_density(Fs[0], na, C_orb)
+                if out is None:
+                    out = self._diag_density(Fs[0], na, with_vectors=True)
+                Pa_n, C_orb = out
                 Pn, Mzn = 2.0 * Pa_n, None
             dP = float(torch.linalg.norm(Pn - P))
             if uhf:
