@@ -529,7 +529,8 @@ struct cq_ctx {
   std::vector<int> cart2shell, sph2shell;
   std::vector<PairInfo> pairs;
   std::vector<double> pairQ;
-  std::vector<int> class_start;      // npc + 1
+  std::vector<int> class_start;      // npc + 1 (two-electron pair list)
+  std::vector<int> class_start_1e;   // npc + 1 (every shell pair, one-electron integrals)
   std::vector<PrimPair> prims;
   std::vector<double> Q;             // nshell x nshell
   double tau = 1e-14;                // include/aointegrals.h:334
@@ -569,7 +570,7 @@ struct cq_ctx {
   DevBuf<ShellMeta> d_shells;
   DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start[2], d_flags;
   DevBuf<double> d_tco, d_boys, d_pairQ, d_Nmax, d_work, d_mats;
-  DevBuf<PairInfo> d_pairs;
+  DevBuf<PairInfo> d_pairs, d_pairs_1e;
   DevBuf<int2> d_s12;
   std::vector<int> grp, cls_grp_off;
   DevBuf<int> d_grp, d_cls_grp_off;
@@ -590,7 +591,7 @@ struct cq_ctx {
   ~cq_ctx() {
     d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
     d_item_start[0].release(); d_item_start[1].release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
-    d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_prims.release();
+    d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_pairs_1e.release(); d_prims.release();
     d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_qcounts.release(); d_nglob.release(); d_pair_nblk.release(); d_pair_nublk.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
@@ -695,6 +696,23 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
         }
       hp.push_back(std::move(h));
     }
+  // the one-electron integrals cover every shell pair; the two-electron lists drop the pairs
+  // without a primitive pair above the threshold (Q = 0: none of their quartets can pass)
+  {
+    std::vector<PairInfo> all;
+    std::vector<int> cls_all;
+    for (auto& h : hp) { all.push_back(h.info); cls_all.push_back(h.cls); }
+    std::vector<int> order(all.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cls_all[x] < cls_all[y]; });
+    std::vector<PairInfo> sorted_all;
+    c->class_start_1e.assign(c->npc + 1, 0);
+    for (int k : order) { sorted_all.push_back(all[k]); c->class_start_1e[cls_all[k] + 1]++; }
+    for (int k = 0; k < c->npc; ++k) c->class_start_1e[k + 1] += c->class_start_1e[k];
+    c->d_pairs_1e.upload(sorted_all, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  hp.erase(std::remove_if(hp.begin(), hp.end(), [](const HostPair& h) { return h.prim.empty(); }), hp.end());
   // group by class (stable)
   std::stable_sort(hp.begin(), hp.end(), [](const HostPair& x, const HostPair& y) { return x.cls < y.cls; });
   c->class_start.assign(c->npc + 1, 0);
@@ -1301,7 +1319,7 @@ int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, 
     for (size_t k = 0; k + 1 < c->prim_off_host.size(); ++k)
       max_prim = std::max(max_prim, c->prim_off_host[k + 1] - c->prim_off_host[k]);
     for (int X = 0; X < c->npc; ++X)
-      launch_onee(X, c->d_pairs.p, c->class_start[X], c->class_start[X + 1], sd, max_prim, dZ, dxyz, natom,
+      launch_onee(X, c->d_pairs_1e.p, c->class_start_1e[X], c->class_start_1e[X + 1], sd, max_prim, dZ, dxyz, natom,
                   c->d_boys.p, c->ncart, Sc, Tc, Vc, s);
     HIPCHK(hipGetLastError());
     double* outs[3] = {S, T, V};
@@ -1405,6 +1423,17 @@ int cq_debug_quartets(cq_ctx_t* c, int nq, const int* shells4, double* out) {
         if (s[k] < 0 || s[k] >= c->nshell) throw std::invalid_argument("shell index out of range");
       auto pid = [&](int x, int y) { return pidx[(size_t)std::max(x, y) * (std::max(x, y) + 1) / 2 + std::min(x, y)]; };
       const int P = pid(s[0], s[1]), Qi = pid(s[2], s[3]);
+      if (P < 0 || Qi < 0) {
+        // a shell pair without any primitive pair above the threshold: the block is zero
+        size_t n = 1;
+        for (int r = 0; r < 4; ++r) {
+          const ShellMeta& sm = c->shells[s[r]];
+          n *= sm.pure ? 2 * sm.l + 1 : ncart_l(sm.l);
+        }
+        std::memset(out + out_off, 0, n * sizeof(double));
+        out_off += n;
+        continue;
+      }
       const PairInfo& pp = c->pairs[P];
       const PairInfo& pq = c->pairs[Qi];
       const int clsP = pair_class(c->shells[pp.a].l, c->shells[pp.b].l);

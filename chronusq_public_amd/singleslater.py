@@ -245,10 +245,12 @@ class SingleSlater:
         for it in range(self.max_iter):
             torch.cuda.synchronize()
             tg = time.perf_counter()
-            if self.incremental and not uhf and it % self.rebuild_every != 0:
-                # FP32 only while the energy still moves by more than 1e-7 Eh per step: near
-                # convergence its rounding would hold the 1e-10 / 1e-8 criteria back
-                if self.precision == "mixed" and abs(last_dE) > 1e-7:
+            # the difference-density build (and FP32 in it) only while the energy still moves by
+            # more than 1e-7 Eh per step: its screening and rounding errors (~1e-9 in PT) would
+            # hold the 1e-10 / 1e-8 criteria back near convergence
+            if (self.incremental and not uhf and it % self.rebuild_every != 0
+                    and abs(last_dE) > 1e-7):
+                if self.precision == "mixed":
                     self.ao.setPrecision("mixed", self.mixed_threshold)
                 dPT, _ = self.formPT(P - P_prev)
                 self.ao.setPrecision("fp64")
