@@ -750,37 +750,69 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_pairs.upload(c->pairs, c->stream);
   c->d_prims.upload(c->prims, c->stream);
   c->d_pairQ.ensure(npair);
-  // Schwarz bounds: diagonal quartets (P|P) per class on the device
-  for (int k = 0; k < kNumClassPairs; ++k) {
-    const ClassKernels& ck = kClassKernels[k];
-    if (ck.i != ck.j) continue;
-    const int X = ck.i;
-    const int n = c->class_start[X + 1] - c->class_start[X];
-    if (n == 0) continue;
-    int la = 0;
-    while ((la + 1) * (la + 2) / 2 <= X) ++la;
-    const int lb = X - la * (la + 1) / 2;
-    const int nab = ncart_l(la) * ncart_l(lb);
-    std::vector<uint2> tasks(n);
-    std::vector<int> ids(n);
-    for (int q = 0; q < n; ++q) {
-      tasks[q] = make_uint2(c->class_start[X] + q, c->class_start[X] + q);
-      ids[q] = c->class_start[X] + q;
+  // Schwarz bounds: diagonal quartets (P|P) per class on the device, the classes on the
+  // auxiliary streams; a pair with more than PRIM_BLOCK primitive pairs is split into
+  // (block i | block j) tasks that add into the pair's output slot
+  {
+    std::vector<uint2> tasks;
+    std::vector<int> slots, ids;
+    struct Launch { int k; long t0, nt; int s0, ns, nab; size_t out0; };
+    std::vector<Launch> launches;
+    size_t out_total = 0;
+    for (int k = 0; k < kNumClassPairs; ++k) {
+      const ClassKernels& ck = kClassKernels[k];
+      if (ck.i != ck.j) continue;
+      const int X = ck.i;
+      const int n = c->class_start[X + 1] - c->class_start[X];
+      if (n == 0) continue;
+      int la = 0;
+      while ((la + 1) * (la + 2) / 2 <= X) ++la;
+      const int lb = X - la * (la + 1) / 2;
+      const int nab = ncart_l(la) * ncart_l(lb);
+      Launch L{k, (long)tasks.size(), 0, (int)ids.size(), n, nab, out_total};
+      for (int q = 0; q < n; ++q) {
+        const int P = c->class_start[X] + q;
+        const int np_ = c->pairs[P].p1 - c->pairs[P].p0;
+        if (np_ > 64 * PRIM_BLOCK) throw std::invalid_argument("more than 512 primitive pairs in one shell pair");
+        const int nb = np_ > PRIM_BLOCK ? (np_ + PRIM_BLOCK - 1) / PRIM_BLOCK : 1;
+        for (int i = 0; i < nb; ++i)
+          for (int j = 0; j < nb; ++j) {
+            const unsigned ci = (unsigned)P | (nb > 1 ? (unsigned)(i + 1) << 24 : 0u);
+            const unsigned cj = (unsigned)P | (nb > 1 ? (unsigned)(j + 1) << 24 : 0u);
+            tasks.push_back(make_uint2(ci, cj));
+            slots.push_back(q);
+          }
+        ids.push_back(P);
+      }
+      L.nt = (long)tasks.size() - L.t0;
+      out_total += (size_t)n * nab * nab;
+      launches.push_back(L);
     }
     DevBuf<uint2> dt;
-    DevBuf<int> di;
+    DevBuf<int> dslot, di;
     DevBuf<double> dblk;
     dt.upload(tasks, c->stream);
+    dslot.upload(slots, c->stream);
     di.upload(ids, c->stream);
-    dblk.ensure((size_t)n * nab * nab);
-    hipLaunchKernelGGL(ck.block, dim3(grid_for(n, WG)), dim3(WG), 0, c->stream, dt.p, (long)n, c->d_pairs.p,
-                       c->d_prims.p, c->d_boys.p, dblk.p, nab * nab);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(schwarz_kernel, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, dblk.p, di.p, n,
-                       c->d_pairs.p, c->d_shells.p, c->d_tco.p, c->d_tco_off.p, c->d_pairQ.p);
-    HIPCHK(hipGetLastError());
+    dblk.ensure(std::max<size_t>(1, out_total));
+    HIPCHK(hipMemsetAsync(dblk.p, 0, std::max<size_t>(1, out_total) * sizeof(double), c->stream));
+    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+    for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
+    int li = 0;
+    for (const Launch& L : launches) {
+      const ClassKernels& ck = kClassKernels[L.k];
+      hipStream_t sa = c->aux[li++ % cq_ctx::kAux];
+      // slots are per class: the slot array entries count from the class's first pair
+      hipLaunchKernelGGL(ck.block, dim3(grid_for(L.nt, WG)), dim3(WG), 0, sa, dt.p + L.t0, L.nt, c->d_pairs.p,
+                         c->d_prims.p, c->d_boys.p, dblk.p + L.out0, L.nab * L.nab, dslot.p + L.t0);
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(schwarz_kernel, dim3(grid_for(L.ns, 128)), dim3(128), 0, sa, dblk.p + L.out0, di.p + L.s0,
+                         L.ns, c->d_pairs.p, c->d_shells.p, c->d_tco.p, c->d_tco_off.p, c->d_pairQ.p);
+      HIPCHK(hipGetLastError());
+    }
+    for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamSynchronize(c->aux[a]));
     HIPCHK(hipStreamSynchronize(c->stream));
-    dt.release(); di.release(); dblk.release();
+    dt.release(); dslot.release(); di.release(); dblk.release();
   }
   c->pairQ.resize(npair);
   HIPCHK(hipMemcpy(c->pairQ.data(), c->d_pairQ.p, npair * sizeof(double), hipMemcpyDeviceToHost));
@@ -1456,8 +1488,9 @@ int cq_debug_quartets(cq_ctx_t* c, int nq, const int* shells4, double* out) {
       DevBuf<double> dout;
       dt.upload(t1, c->stream);
       dout.ensure(blk);
+      HIPCHK(hipMemsetAsync(dout.p, 0, blk * sizeof(double), c->stream));
       hipLaunchKernelGGL(kClassKernels[kk].block, dim3(1), dim3(WG), 0, c->stream, dt.p, 1L, c->d_pairs.p,
-                         c->d_prims.p, c->d_boys.p, dout.p, blk);
+                         c->d_prims.p, c->d_boys.p, dout.p, blk, (const int*)nullptr);
       HIPCHK(hipGetLastError());
       std::vector<double> h(blk);
       HIPCHK(hipMemcpyAsync(h.data(), dout.p, blk * sizeof(double), hipMemcpyDeviceToHost, c->stream));

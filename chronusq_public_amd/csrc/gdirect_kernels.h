@@ -184,10 +184,12 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       }                                                                                       \
     }                                                                                         \
   }                                                                                           \
+  /* Block kernel: task t adds its (primitive-block) contribution to the contracted Cartesian \
+     block of output slot slot[t] (t when slot is null), zeroed by the caller. */              \
   __global__ void __launch_bounds__(WG) eri_block_##NAME(                                       \
       const uint2* __restrict__ tasks, long count, const PairInfo* __restrict__ pairs,          \
       const PrimPair* __restrict__ prims, const double* __restrict__ boys, double* out,          \
-      int blk) {                                                                                \
+      int blk, const int* __restrict__ slot) {                                                  \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                        \
     if ((long)blockIdx.x * WG >= count) return;                                                 \
     stage_boys(boys, LTOT, boys_lds);                                                           \
@@ -197,7 +199,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       const long t = base + threadIdx.x;                                                        \
       QuartetArgs q;                                                                            \
       make_quartet(tasks, t, count, wbase, pairs, prims, q);                                    \
-      if (t < count) quartet_block_##NAME(q, out + t * (long)blk, boys_lds);                   \
+      if (t < count) quartet_block_##NAME(q, out + (slot ? (long)slot[t] : t) * (long)blk, boys_lds); \
     }                                                                                           \
   }
 
@@ -270,7 +272,8 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
 
 typedef void (*JKKernel)(const uint2*, const unsigned long long*, const PairInfo*, const PrimPair*,
                          const double*, JKArgs);
-typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int);
+typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int,
+                            const int*);
 typedef void (*SegKernel)(const uint2*, const Segment*, const unsigned*, unsigned*, const PairInfo*, const PrimPair*,
                           const double*, JKArgs);
 

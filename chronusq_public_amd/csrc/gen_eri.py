@@ -337,7 +337,7 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                 for ib in range(NB):
                     for (ic, idd) in ks:
                         idx = ((ia * NB + ib) * NC + ic) * ND + idd
-                        em(f"out[{idx}] = o_{ia}_{ib}_{ic}_{idd};")
+                        em(f"atomicAdd(&out[{idx}], o_{ia}_{ib}_{ic}_{idd});")
         else:
             gen_digest_chunk(em, NA, NB, NC, ND, ks, persist)
         em.ind -= 1

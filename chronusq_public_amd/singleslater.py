@@ -109,9 +109,8 @@ class SingleSlater:
         self.S, self.H = S, T + V
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        w, U = torch.linalg.eigh(S)
-        self.X = (U * w.rsqrt()) @ U.T          # S^{-1/2} (Loewdin, aointegrals_builders.cpp:736-769)
-        self.Xi = (U * w.sqrt()) @ U.T          # S^{1/2}
+        # S^{-1/2} (Loewdin, aointegrals_builders.cpp:736-769) and S^{1/2}
+        self.X, self.Xi = self._lowdin(S)
         torch.cuda.synchronize()
         self.setup_parts = {"context_s": t1 - t0, "one_e_s": t2 - t1, "lowdin_s": time.perf_counter() - t2}
         self.enuc = mol.nuclear_repulsion()
@@ -127,6 +126,29 @@ class SingleSlater:
         self.setup_s = time.perf_counter() - t0
 
     # ------------------------------------------------------------------ pieces
+    def _lowdin(self, S):
+        """S^{-1/2}, S^{1/2} by the coupled Newton-Schulz iteration on A = S / c (c >= the
+        largest eigenvalue, Gershgorin): Y <- Y T, Z <- T Z, T = (3 I - Z Y) / 2 converges
+        quadratically to A^{1/2}, A^{-1/2} (GEMMs only; the eigensolve is launch-bound on the
+        GPU).  Falls back to the eigendecomposition if the residual |Z Y - I| stays above
+        1e-12 (a nearly singular S)."""
+        torch = self.torch
+        n = S.shape[0]
+        eye = torch.eye(n, dtype=S.dtype, device=S.device)
+        c = float(S.abs().sum(dim=1).max())
+        Y, Z = S / c, eye.clone()
+        for it in range(60):
+            T = 1.5 * eye - 0.5 * (Z @ Y)
+            Y, Z = Y @ T, T @ Z
+            if it % 6 == 5 and float((Z @ Y - eye).abs().max()) < 1e-13:
+                break
+        if float((Z @ Y - eye).abs().max()) < 1e-12:
+            Z = 0.5 * (Z + Z.T)
+            Y = 0.5 * (Y + Y.T)
+            return Z / c ** 0.5, Y * c ** 0.5
+        w, U = torch.linalg.eigh(S)
+        return (U * w.rsqrt()) @ U.T, (U * w.sqrt()) @ U.T
+
     def formPT(self, P, Mz=None):
         torch = self.torch
         N = self.N
