@@ -116,3 +116,16 @@ def test_pseudo_diagonalisation_matches_eigensolve():
     assert pd.converged
     assert abs(pd.energy - ref.energy) < 1e-9
     assert pd.iterations <= ref.iterations + 2
+
+
+def test_input_deck_runs_to_golden_energy():
+    """The reference's test0001 deck (water RHF/STO-3G) through the job driver."""
+    import io
+    from chronusq_public_amd.run import run_deck
+    with open(os.path.join(GOLD, "energies.json")) as f:
+        gold = json.load(f)
+    out = io.StringIO()
+    r = run_deck(os.path.join(GOLD, "decks", "water_sto3g.inp"), out=out)
+    e_ref = gold["tests"]["test0001"]["E"]
+    assert r["converged"] and abs(r["energy"] - e_ref) < 5e-10
+    assert "Total Energy" in out.getvalue()
