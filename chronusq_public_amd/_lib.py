@@ -27,6 +27,10 @@ EXPORTED = [
     "cq_contract_direct", "cq_contract_direct_device", "cq_get_quartet_counts",
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
     "cq_get_kernel_times", "cq_get_work_stats", "cq_one_electron", "cq_set_launch_streams", "cq_set_lds_digestion",
+    "cq_solid_harmonics",
+    # learned initial guess (include/cqgpu_guess.h)
+    "cq_mlp_last_error", "cq_mlp_gemm_bf16", "cq_mlp_relu_backward", "cq_mlp_column_sum", "cq_mlp_to_bf16",
+    "cq_mlp_mse", "cq_mlp_adam",
 ]
 
 _lib = None
@@ -75,11 +79,20 @@ def load():
     L.cq_one_electron.argtypes = [vp, ctypes.c_int, dp, dp, vp, vp, vp, ctypes.c_int, vp]
     L.cq_get_kernel_times.argtypes = [vp, dp, P(ctypes.c_longlong), ctypes.c_int]
     L.cq_get_work_stats.argtypes = [vp, dp, ctypes.c_int]
+    L.cq_solid_harmonics.argtypes = [ctypes.c_int, dp]
+    ci, cl, cf = ctypes.c_int, ctypes.c_long, ctypes.c_float
+    L.cq_mlp_last_error.argtypes = []
+    L.cq_mlp_last_error.restype = ctypes.c_char_p
+    L.cq_mlp_gemm_bf16.argtypes = [vp, ci, ci, vp, ci, ci, vp, ci, vp, ci, ci, ci, vp, ci, ci, vp]
+    L.cq_mlp_relu_backward.argtypes = [vp, vp, vp, vp, cl, vp]
+    L.cq_mlp_column_sum.argtypes = [vp, ci, ci, vp, vp]
+    L.cq_mlp_to_bf16.argtypes = [vp, vp, cl, vp]
+    L.cq_mlp_mse.argtypes = [vp, vp, vp, vp, ci, ci, cf, vp, vp, vp]
+    L.cq_mlp_adam.argtypes = [vp, vp, vp, vp, vp, cl, cf, cf, cf, cf, ci, cf, vp]
     for name in EXPORTED:
         fn = getattr(L, name)
-        if fn.restype is ctypes.c_int or name not in ("cq_last_error",):
-            if name != "cq_last_error":
-                fn.restype = ctypes.c_int
+        if name not in ("cq_last_error", "cq_mlp_last_error"):
+            fn.restype = ctypes.c_int
     _lib = L
     return L
 

@@ -41,7 +41,8 @@ def build(incremental: bool = False, resource_usage: bool = False, jobs: int = 0
     os.makedirs(BUILD, exist_ok=True)
     gen_script = os.path.join(CSRC, "gen_eri.py")
     subprocess.run([sys.executable, gen_script, GEN], check=True)
-    srcs = sorted(glob.glob(os.path.join(GEN, "*.hip"))) + [os.path.join(CSRC, "gdirect.hip"), os.path.join(CSRC, "onee.hip")]
+    srcs = sorted(glob.glob(os.path.join(GEN, "*.hip"))) + [os.path.join(CSRC, "gdirect.hip"), os.path.join(CSRC, "onee.hip"),
+                                                              os.path.join(CSRC, "mlp.hip")]
     deps = [os.path.join(CSRC, f) for f in ("gdirect_device.h", "gdirect_kernels.h")] + [gen_script]
     dep_mtime = max(os.path.getmtime(d) for d in deps)
     header_mtime = os.path.getmtime(os.path.join(HERE, "..", "include", "cqgpu.h"))

@@ -1319,6 +1319,17 @@ int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld_) {
 
 int cq_num_class_pairs(void) { return kNumClassPairs; }
 
+// Coefficients of the real solid harmonics of angular momentum l (m = -l..l, row-major
+// (2l+1) x ncart(l)) in the uniformly normalised Cartesian functions of a shell: the transform
+// every pure shell uses (the reference's cart2sphCoeff / makeCar2Sph,
+// src/basisset/basisset_read.cpp:871-990).
+int cq_solid_harmonics(int l, double* out) {
+  if (l < 0 || l > 2 || !out) return CQ_ERR_INVALID;
+  const std::vector<double> M = solid_harmonics(l);
+  std::copy(M.begin(), M.end(), out);
+  return CQ_OK;
+}
+
 // S, T, V in the shells' basis (column-major n x n, leading dimension ld, device memory),
 // replacing AOIntegrals::computeAOOneE (src/aointegrals/aointegrals_builders.cpp:337-467).
 // Z[natom] nuclear charges and xyz[3*natom] positions (bohr) are host arrays.

@@ -97,6 +97,11 @@ int cq_contract_direct_device(cq_ctx_t* ctx, int nmat, const double* const* X, d
 int cq_one_electron(cq_ctx_t* ctx, int natom, const double* Z, const double* xyz, double* S, double* T,
                     double* V, int ld, void* stream);
 
+/* Real solid-harmonic coefficients for angular momentum l (0..2): (2l+1) x ncart(l) row-major,
+ * m = -l..l, in the shell's uniformly normalised Cartesian functions (x^l ... z^l) -- the
+ * transform of every pure shell (cart2sphCoeff, src/basisset/basisset_read.cpp:871-990). */
+int cq_solid_harmonics(int l, double* out);
+
 /* Surviving quartets per class pair of the last contraction (synchronises). */
 int cq_get_quartet_counts(cq_ctx_t* ctx, long long* counts, int max_counts, int* n_out);
 int cq_num_class_pairs(void);

@@ -80,8 +80,11 @@ def test_alkane_composition():
 
 
 def _header_functions():
-    txt = open(os.path.join(ROOT, "include", "cqgpu.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(cq_\w+)\s*\(", txt, re.M)))
+    names = set()
+    for h in ("cqgpu.h", "cqgpu_guess.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"^\s*(?:int|const char\*)\s+(cq_\w+)\s*\(", txt, re.M))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
