@@ -154,32 +154,43 @@ def main():
 
     # ---- SCF time-to-energy (device SCF: 1e integrals, SAD guess, CDIIS, eigensolve, G[D])
     from chronusq_public_amd.singleslater import clear_sad_cache
-    w = M.water_monomer()   # warm-up: code objects, solver handles
+    w = M.water_monomer()   # warm-up: code objects, solver handles, the guess network's weights
     SingleSlater(w, "cc-pvdz", tau=args.tau).SCF3()
+    SingleSlater(M.water_cluster(2, seed=1, box_ang=M.scaled_box(2)), "cc-pvdz", tau=args.tau,
+                 guess="LEARNED").SCF3()
 
-    def run_scf(inc, cold, precision="fp64"):
+    def run_scf(inc, cold, precision="fp64", guess="SAD", judge=True):
         if cold:
             clear_sad_cache()   # the SAD guess recomputes its atomic UHF densities
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = SingleSlater(mol, bs, tau=args.tau, guess="SAD", incremental=inc, rebuild_every=8,
-                         precision=precision, mixed_threshold=1e-5).SCF3()
+        r = SingleSlater(mol, bs, tau=args.tau, guess=guess, incremental=inc, rebuild_every=8,
+                         precision=precision, mixed_threshold=1e-5, judge_guess=judge).SCF3()
         torch.cuda.synchronize()
         return r, time.perf_counter() - t0
 
     r, wall = run_scf(False, True)
     E_ref, P_conv = r.energy, r.P
     scf = {}
-    for name, inc, prec in (("fp64", False, "fp64"), ("fp64_incremental", True, "fp64"),
-                            ("mixed_incremental", True, "mixed")):
-        for cold in (True, False):
-            r, wall = run_scf(inc, cold, prec)
-            key = name + ("_cold_sad" if cold else "")
-            scf[key] = {"energy": r.energy, "dE_vs_fp64": r.energy - E_ref, "iterations": r.iterations,
-                        "wall_s": wall, "setup_s": r.setup_s, "setup_parts": r.setup_parts,
-                        "guess_s": r.guess_s,
-                        "time_to_1e-6_s": r.setup_s + time_to_energy(r, E_ref, 1e-6),
-                        "gd_ms_mean": 1e3 * float(np.mean(r.gd_times))}
+    # SAD (cold: atomic UHF densities recomputed; warm: cached per process) and the learned
+    # guess (network trained on small water clusters and alkanes; "judged": the SAD start is
+    # made too and the lower-energy start kept)
+    variants = [(name + ("_cold_sad" if cold else ""), inc, prec, "SAD", True, cold)
+                for name, inc, prec in (("fp64", False, "fp64"), ("fp64_incremental", True, "fp64"),
+                                        ("mixed_incremental", True, "mixed"))
+                for cold in (True, False)]
+    variants += [("fp64_learned", False, "fp64", "LEARNED", False, False),
+                 ("fp64_learned_judged", False, "fp64", "LEARNED", True, False),
+                 ("mixed_incremental_learned", True, "mixed", "LEARNED", False, False),
+                 ("mixed_incremental_learned_judged", True, "mixed", "LEARNED", True, False)]
+    for key, inc, prec, guess, judge, cold in variants:
+        r, wall = run_scf(inc, cold, prec, guess, judge)
+        scf[key] = {"energy": r.energy, "dE_vs_fp64": r.energy - E_ref, "iterations": r.iterations,
+                    "wall_s": wall, "setup_s": r.setup_s, "setup_parts": r.setup_parts,
+                    "guess_s": r.guess_s, "guess_used": str(r.guess_info.get("used")),
+                    "first_iteration_error": r.energies[0] - E_ref,
+                    "time_to_1e-6_s": r.setup_s + time_to_energy(r, E_ref, 1e-6),
+                    "gd_ms_mean": 1e3 * float(np.mean(r.gd_times))}
 
     if args.calibrate_copy:
         src = torch.empty(32 << 20, dtype=torch.float64, device=dev).fill_(1.0)

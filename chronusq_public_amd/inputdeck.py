@@ -152,8 +152,9 @@ def parse_deck_text(text: str) -> Deck:
     if "FIELD" in scf and any(abs(x) > 0 for x in scf["FIELD"]):
         raise DeckError("static fields are not supported by this SCF driver")
     guess = scf.get("GUESS", "SAD").upper()
-    if guess not in ("SAD", "CORE"):
-        raise DeckError(f"guess {guess} is not supported (SAD or CORE)")
+    # LEARNED (the network guess of learned_guess.py) has no reference counterpart
+    if guess not in ("SAD", "CORE", "LEARNED"):
+        raise DeckError(f"guess {guess} is not supported (SAD, CORE or LEARNED)")
     if mol.natom == 1:
         guess = "CORE"           # the reference defaults to CORE for one atom (parseQM.py:84-87)
     alg = v.get("INTS", {}).get("ALG", "DIRECT").upper()

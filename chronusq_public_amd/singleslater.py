@@ -71,6 +71,7 @@ class SCFResult:
     setup_s: float = 0.0
     setup_parts: Dict[str, float] = field(default_factory=dict)
     guess_s: float = 0.0
+    guess_info: Dict[str, object] = field(default_factory=dict)
 
 
 class SingleSlater:
@@ -79,7 +80,8 @@ class SingleSlater:
                  max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None,
                  incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True,
                  precision: str = "fp64", mixed_threshold: float = 1e-5,
-                 pseudo_diag: bool = True, pdiag_switch: float = 1e-1):
+                 pseudo_diag: bool = True, pdiag_switch: float = 1e-1,
+                 guess_model: Optional[str] = None, judge_guess: bool = True):
         import torch
         self.torch = torch
         self.mol = mol
@@ -93,6 +95,10 @@ class SingleSlater:
         # singleslater_scfutils.h:423-564)
         self.incremental, self.rebuild_every = incremental, rebuild_every
         self.sad_cache = sad_cache
+        # learned guess: network file (default: the one trained for this basis) and whether
+        # the SAD density competes with it on energy
+        self.guess_model, self.judge_guess = guess_model, judge_guess
+        self.guess_info: Dict[str, object] = {}
         self.ene_tol, self.den_tol, self.max_iter = ene_tol, den_tol, max_iter
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         t0 = time.perf_counter()
@@ -207,12 +213,14 @@ class SingleSlater:
         Pb, _ = self._diag_density(self.H, nb)
         return Pa + Pb, Pa - Pb
 
-    def sad_guess(self):
-        """Superposition of atomic (UHF) densities, spin-averaged, then one Fock build and
-        diagonalisation (singleslater_guess.h:24-65,99-222)."""
+    def _atomic_densities(self, spherical: bool = False):
+        """Block-diagonal superposition of atomic (UHF) densities, spin-averaged, from the
+        per-(Z, multiplicity, basis, device) cache (singleslater_guess.h:99-222).  With
+        spherical, every atomic block is averaged over rotations (the learned guess's
+        baseline, which must not depend on the molecule's orientation): a block between two
+        shells of one l becomes tr/(2l+1) times the identity, blocks between different l vanish
+        (Schur's lemma for the shells' irreducible rotations); electron counts are kept."""
         torch = self.torch
-        if self.mol.natom <= 1:
-            return self.core_guess()
         atom_of_bf = np.concatenate([[s.atom] * s.size for s in self.basis.shells])
         Ps = torch.zeros((self.N, self.N), dtype=torch.float64, device=self.dev)
         cache: Dict[int, object] = {}
@@ -228,11 +236,35 @@ class SingleSlater:
             cache[z] = r.P
             if self.sad_cache:
                 _SAD_ATOMS[key] = r.P
+        if spherical:
+            for z in list(cache):
+                key = ("spherical", z, DEFAULT_MULT.get(z, 1), self.basis.name, str(self.dev))
+                if key not in _SAD_ATOMS:
+                    _SAD_ATOMS[key] = self._spherical_average(z, cache[z])
+                cache[z] = _SAD_ATOMS[key]
         for iat, z in enumerate(self.mol.Z):
             idx = np.nonzero(atom_of_bf == iat)[0]
             lo, hi = int(idx[0]), int(idx[-1]) + 1
             Ps[lo:hi, lo:hi] = cache[int(z)]
-        PT, _ = self.formPT(Ps)
+        return Ps
+
+    def _spherical_average(self, z: int, blk):
+        iat = int(np.nonzero(self.mol.Z == z)[0][0])
+        shells = [(s.l, s.pure, s.size) for s in self.basis.shells if s.atom == iat]
+        if any(l > 1 and not pure for (l, pure, _) in shells):
+            raise ValueError("spherical averaging needs pure shells above l = 1")
+        out = self.torch.zeros_like(blk)
+        offs = np.concatenate([[0], np.cumsum([n for (_, _, n) in shells])])
+        for a, (la, _, na) in enumerate(shells):
+            for b, (lb, _, nb) in enumerate(shells):
+                if la != lb:
+                    continue
+                sub = blk[offs[a]:offs[a] + na, offs[b]:offs[b] + nb]
+                out[offs[a]:offs[a] + na, offs[b]:offs[b] + nb] = (
+                    self.torch.eye(na, dtype=blk.dtype, device=blk.device) * (self.torch.trace(sub) / na))
+        return out
+
+    def _guess_from_fock(self, PT):
         F = self.H + 0.5 * PT
         na, nb = self.nocc
         Pa, _ = self._diag_density(F, na)
@@ -241,14 +273,69 @@ class SingleSlater:
         Pb, _ = self._diag_density(F, nb)
         return Pa + Pb, Pa - Pb
 
+    def sad_guess(self):
+        """Superposition of atomic (UHF) densities, spin-averaged, then one Fock build and
+        diagonalisation (singleslater_guess.h:24-65,99-222)."""
+        self.guess_info = {"used": "SAD"}
+        if self.mol.natom <= 1:
+            self.guess_info["used"] = "CORE"
+            return self.core_guess()
+        PT, _ = self.formPT(self._atomic_densities())
+        return self._guess_from_fock(PT)
+
+    def learned_guess(self):
+        """Atomic densities plus network-predicted pair blocks and block corrections
+        (learned_guess.py), then one Fock build and diagonalisation as for SAD.  With
+        judge_guess the SAD starting density is made as well, and of the two idempotent
+        starting densities the one with the lower (variational) energy E[P0] = Tr(P0 H) +
+        1/4 Tr(P0 PT[P0]) + E_nuc is kept; its PT[P0] becomes the SCF's first Fock build, so
+        judging costs two extra G[D] builds.  Molecules or bases the network does not cover
+        take the SAD guess.  No reference counterpart."""
+        from . import learned_guess as LG
+        if self.mol.natom <= 1:
+            return self.sad_guess()
+        try:
+            net = LG.load_model(self.basis.name, path=self.guess_model, device=self.dev)
+            PL = LG.predict_density(net, self.mol, self.basis, self._atomic_densities(spherical=True))
+        except LG.GuessUnavailable as e:
+            out = self.sad_guess()
+            self.guess_info["fallback"] = str(e)
+            return out
+        uhf = self.ref != "RHF"
+        P0_L = self._guess_from_fock(self.formPT(PL)[0])
+        self.guess_info = {"used": "LEARNED"}
+        if not self.judge_guess:
+            return P0_L
+        P0_S = self._guess_from_fock(self.formPT(self._atomic_densities())[0])
+        cand = []
+        for name, (P0, Mz0) in (("LEARNED", P0_L), ("SAD", P0_S)):
+            PT0, PTz0 = self.formPT(P0, Mz0 if uhf else None)
+            cand.append((self._energy(P0, PT0, Mz0, PTz0), name, P0, Mz0, PT0, PTz0))
+        self.guess_info.update({"E0_learned": cand[0][0], "E0_sad": cand[1][0]})
+        E0, name, P0, Mz0, PT0, PTz0 = min(cand, key=lambda c: c[0])
+        self.guess_info["used"] = name
+        self._first_PT = (P0, PT0, PTz0)
+        return P0, Mz0
+
     # ------------------------------------------------------------------ SCF3
     def SCF3(self, P0=None, Mz0=None, reference_energy: Optional[float] = None) -> SCFResult:
         torch = self.torch
         t_start = time.perf_counter()
+        self._first_PT = None
         if P0 is None:
-            P, Mz = self.sad_guess() if self.guess == "SAD" else self.core_guess()
+            if self.guess == "SAD":
+                P, Mz = self.sad_guess()
+            elif self.guess == "LEARNED":
+                P, Mz = self.learned_guess()
+            elif self.guess == "CORE":
+                P, Mz = self.core_guess()
+                self.guess_info = {"used": "CORE"}
+            else:
+                raise ValueError(f"unknown guess {self.guess}")
         else:
+            self.guess_info = {"used": "INPUT"}
             P, Mz = P0, Mz0
+        first_PT, self._first_PT = self._first_PT, None
         torch.cuda.synchronize()
         guess_s = time.perf_counter() - t_start
         uhf = self.ref != "RHF"
@@ -256,7 +343,7 @@ class SingleSlater:
         hist_F: List[tuple] = []
         hist_e: List[torch.Tensor] = []
         res = SCFResult(energy=0.0, iterations=0, converged=False, setup_s=self.setup_s, guess_s=guess_s,
-                        setup_parts=dict(self.setup_parts))
+                        setup_parts=dict(self.setup_parts), guess_info=dict(self.guess_info))
         E_old = 0.0
         last_dE = float("inf")
         C_orb = None    # orthonormal-basis orbitals of the last density step (RHF)
@@ -270,7 +357,9 @@ class SingleSlater:
             # the difference-density build (and FP32 in it) only while the energy still moves by
             # more than 1e-7 Eh per step: its screening and rounding errors (~1e-9 in PT) would
             # hold the 1e-10 / 1e-8 criteria back near convergence
-            if (self.incremental and not uhf and it % self.rebuild_every != 0
+            if it == 0 and first_PT is not None and first_PT[0] is P:
+                PT, PTz = first_PT[1], first_PT[2]
+            elif (self.incremental and not uhf and it % self.rebuild_every != 0
                     and abs(last_dE) > 1e-7):
                 if self.precision == "mixed":
                     self.ao.setPrecision("mixed", self.mixed_threshold)
