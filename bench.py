@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-12)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-molecule SCF")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run three 256 MiB device-to-device copies (a plain copy kernel of "
                          "known size, the scale for HBM byte counters)")
@@ -191,6 +192,43 @@ def main():
                     "first_iteration_error": r.energies[0] - E_ref,
                     "time_to_1e-6_s": r.setup_s + time_to_energy(r, E_ref, 1e-6),
                     "gd_ms_mean": 1e3 * float(np.mean(r.gd_times))}
+
+    # ---- batched uneven molecules: 128 alkanes C_nH_{2n+2}, n in [1, 12], and 128 (H2O)_m,
+    # m in [1, 8], 6-31G*, one batched SCF; the same molecules one by one for comparison
+    batch = None
+    if not args.no_batch:
+        from chronusq_public_amd.batch import BatchSCF, c4_molecules
+        mols4 = c4_molecules(args.seed)
+        BatchSCF(mols4[:4] + mols4[-4:], "6-31g*").run()      # warm-up: code objects, atoms
+        batch = {"molecules": len(mols4), "basis": "6-31g*"}
+        for key, prec, inc in (("fp64", "fp64", False), ("mixed_incremental", "mixed", True)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bs4 = BatchSCF(mols4, "6-31g*", tau=args.tau, precision=prec, incremental=inc)
+            rb = bs4.run()
+            torch.cuda.synchronize()
+            wall4 = time.perf_counter() - t0
+            bs4.close()
+            batch[key] = {"molecules_per_s": len(mols4) / wall4, "wall_s": wall4, "setup_s": rb.setup_s,
+                          "per_molecule_s_p50": float(np.nanmedian(rb.seconds)),
+                          "per_molecule_s_p99": float(np.nanpercentile(rb.seconds, 99)),
+                          "iterations_max": int(rb.iterations.max()), "converged": int(rb.converged.sum())}
+            if key == "fp64":
+                E4 = rb.energies
+        nseq = 32
+        seq_t = []
+        dE4 = 0.0
+        for k in range(nseq):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rs4 = SingleSlater(mols4[k * len(mols4) // nseq], "6-31g*", tau=args.tau).SCF3()
+            torch.cuda.synchronize()
+            seq_t.append(time.perf_counter() - t0)
+            dE4 = max(dE4, abs(rs4.energy - E4[k * len(mols4) // nseq]))
+        batch["one_by_one"] = {"molecules_per_s": nseq / sum(seq_t), "sample": f"{nseq} of the molecules, evenly spaced",
+                               "per_molecule_s_p50": float(np.median(seq_t)),
+                               "per_molecule_s_p99": float(np.percentile(seq_t, 99)),
+                               "max_abs_dE_vs_batch": dE4}
 
     if args.calibrate_copy:
         src = torch.empty(32 << 20, dtype=torch.float64, device=dev).fill_(1.0)
@@ -281,6 +319,7 @@ def main():
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC_BYTES.get(dom["class"])},
         "scf": scf,
+        "batch": batch,
         "classes": sorted(per_class, key=lambda r: -r["kernel_ms"]),
         "wall_s_timed": wall,
     }

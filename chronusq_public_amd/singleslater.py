@@ -54,6 +54,21 @@ def clear_sad_cache():
     _SAD_ATOMS.clear()
 
 
+def atomic_density(z: int, basis_name: str, dev, ndiis: int = 10, use_cache: bool = True):
+    """Spin-summed density of the free atom (UHF at its default multiplicity, CORE guess, from
+    the per-(Z, multiplicity, basis, device) cache) -- the SAD guess's atomic block
+    (singleslater_guess.h:99-222)."""
+    key = (z, DEFAULT_MULT.get(z, 1), basis_name, str(dev))
+    if use_cache and key in _SAD_ATOMS:
+        return _SAD_ATOMS[key]
+    atom = Molecule(np.array([z]), np.zeros((1, 3)), 0, DEFAULT_MULT.get(z, 1))
+    P = SingleSlater(atom, basis_name, ref="UHF", tau=1e-14, guess="CORE", ndiis=ndiis, max_iter=512,
+                     device=dev).SCF3().P
+    if use_cache:
+        _SAD_ATOMS[key] = P
+    return P
+
+
 class SCFNotConverged(RuntimeError):
     """SCF failed to converge within maxSCFIter (singleslater_scf.h:253-255)."""
 
@@ -223,19 +238,8 @@ class SingleSlater:
         torch = self.torch
         atom_of_bf = np.concatenate([[s.atom] * s.size for s in self.basis.shells])
         Ps = torch.zeros((self.N, self.N), dtype=torch.float64, device=self.dev)
-        cache: Dict[int, object] = {}
-        for z in dict.fromkeys(int(v) for v in self.mol.Z):
-            key = (z, DEFAULT_MULT.get(z, 1), self.basis.name, str(self.dev))
-            if self.sad_cache and key in _SAD_ATOMS:
-                cache[z] = _SAD_ATOMS[key]
-                continue
-            atom = Molecule(np.array([z]), np.zeros((1, 3)), 0, DEFAULT_MULT.get(z, 1))
-            hf = SingleSlater(atom, self.basis.name, ref="UHF", tau=1e-14, guess="CORE", ndiis=self.ndiis,
-                              max_iter=512, device=self.dev)
-            r = hf.SCF3()
-            cache[z] = r.P
-            if self.sad_cache:
-                _SAD_ATOMS[key] = r.P
+        cache: Dict[int, object] = {z: atomic_density(z, self.basis.name, self.dev, self.ndiis, self.sad_cache)
+                                    for z in dict.fromkeys(int(v) for v in self.mol.Z)}
         if spherical:
             for z in list(cache):
                 key = ("spherical", z, DEFAULT_MULT.get(z, 1), self.basis.name, str(self.dev))
