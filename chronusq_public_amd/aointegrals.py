@@ -89,6 +89,53 @@ class AOIntegrals:
             raise ValueError(f"unknown precision mode {mode!r}")
         check(_lib.load().cq_set_precision_mode(self._h, code, float(threshold)), self._h)
 
+    def autotune(self, X, reps: int = 3, variants: Sequence[int] = (0, 1, 2)) -> dict:
+        """Pick, per class pair, the fastest J/K digestion variant for the RHF contraction of
+        the device density X (0: FP64 atomics into HBM, 1: ket-keyed LDS segments, 2: also
+        bra-keyed segments where the class pair allows them).  Every variant runs `reps`
+        times with the class kernels serialised and timed by HIP events
+        (cq_get_kernel_times); the fastest is set per class pair (cq_set_class_digestion).
+        Returns {class name: (chosen variant, {variant: mean ms})}.  No reference
+        counterpart."""
+        import torch
+        L = _lib.load()
+        nc = L.cq_num_class_pairs()
+        J = torch.zeros_like(X)
+        K = torch.zeros_like(X)
+        times = {}
+        check(L.cq_set_launch_streams(self._h, 1), self._h)
+        try:
+            for v in variants:
+                for k in range(nc):
+                    check(L.cq_set_class_digestion(self._h, k, int(v)), self._h)
+                self.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+                check(L.cq_set_profiling(self._h, 1), self._h)
+                for _ in range(reps):
+                    self.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+                ms = (ctypes.c_double * nc)()
+                nl = (ctypes.c_longlong * nc)()
+                check(L.cq_get_kernel_times(self._h, ms, nl, nc), self._h)
+                check(L.cq_set_profiling(self._h, 0), self._h)
+                times[int(v)] = [ms[k] / nl[k] if nl[k] else 0.0 for k in range(nc)]
+        finally:
+            check(L.cq_set_launch_streams(self._h, 4), self._h)
+        names = list(self.quartet_counts().keys())
+        out = {}
+        for k in range(nc):
+            best = min(times, key=lambda v: times[v][k])
+            check(L.cq_set_class_digestion(self._h, k, best), self._h)
+            out[names[k]] = (best, {v: times[v][k] for v in times})
+        return out
+
+    def class_digestion(self) -> List[int]:
+        L = _lib.load()
+        out = []
+        for k in range(L.cq_num_class_pairs()):
+            v = ctypes.c_int()
+            check(L.cq_get_class_digestion(self._h, k, ctypes.byref(v)), self._h)
+            out.append(v.value)
+        return out
+
     def computeSchwartz(self) -> np.ndarray:
         Q = np.zeros((self.nShell, self.nShell))
         check(_lib.load().cq_get_schwarz(self._h, _dp(Q)), self._h)

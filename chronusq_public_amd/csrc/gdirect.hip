@@ -584,6 +584,8 @@ struct cq_ctx {
   DevBuf<Segment> d_segs;
   DevBuf<unsigned> d_nseg, d_work_q;
   int lds_mode = 2;   // 0 HBM atomics, 1 ket-keyed LDS segments, 2 also bra-keyed segments
+  // per class pair: the digestion variant (0..2 as lds_mode, capped by it), -1 = lds_mode
+  std::vector<int> class_variant = std::vector<int>(CQ_NUM_CLASS_PAIRS, -1);
   DevBuf<unsigned long long> d_counts, d_qcounts, d_nglob;
   DevBuf<unsigned char> d_pair_nblk, d_pair_nublk;
   std::string err;
@@ -937,13 +939,16 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     }
   }
   c->task_capacity = off;
+  // one call may take table 0 for some class pairs and table 1 for others (per-class digestion
+  // variants): table 1's segments live after table 0's
+  for (auto& mt : c->cps[1]) mt.seg_off += seg_total[0];
   for (int t = 0; t < 2; ++t) {
     c->nitem[t] = items[t];
     c->d_cps[t].upload(c->cps[t], c->stream);
     c->d_item_start[t].upload(c->item_start[t], c->stream);
   }
   c->d_tasks.ensure((size_t)std::max<long long>(1, off));
-  c->d_segs.ensure((size_t)std::max<long long>(1, std::max(seg_total[0], seg_total[1])));
+  c->d_segs.ensure((size_t)std::max<long long>(1, seg_total[0] + seg_total[1]));
   c->d_nseg.ensure(kNumClassPairs);
   c->d_work_q.ensure(kNumClassPairs);
   c->d_counts.ensure(kNumClassPairs);
@@ -1031,9 +1036,12 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
                        base, c->d_flags.p + k);
     HIPCHK(hipGetLastError());
   }
-  // bra-keyed segments need the segment kernels for every pass: one density slot, LDS on
-  const int tb = (c->lds_mode >= 2 && nslot == 1) ? 1 : 0;
-  const std::vector<CPMeta>& cps = c->cps[tb];
+  // digestion variant per class pair (the lds_mode, or the class's own choice below it);
+  // bra-keyed segments (task table 1) need one density slot
+  auto variant_of = [&](int k) {
+    const int v = c->class_variant[k] < 0 ? c->lds_mode : std::min(c->class_variant[k], c->lds_mode);
+    return v;
+  };
   // mixed precision: a second round screens the bound window [tau, threshold) into the same
   // task buffers and runs it with FP32 integrals (one density slot on the segment path only)
   const bool mixed = c->precision_mode == CQ_PREC_MIXED && nslot == 1 && c->lds_mode >= 1 &&
@@ -1081,7 +1089,9 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
     for (int kk = kNumClassPairs - 1; kk >= 0; --kk) {
       const int k = c->launch_order[kk];
       const ClassKernels& ck = kClassKernels[k];
-      const CPMeta& m = cps[k];
+      const int var = variant_of(k);
+      const int tb = (var >= 2 && nslot == 1) ? 1 : 0;
+      const CPMeta& m = c->cps[tb][k];
       if (m.nbra == 0 || m.nket == 0) continue;
       hipStream_t sk = c->aux[launched++ % c->n_streams];
       if (s0 == 0) {
@@ -1099,7 +1109,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
       // one density slot and LDS room for the K columns: segment kernel (LDS accumulation)
       const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->ncart + (size_t)ck.nc * ck.nd) * sizeof(double);
-      const bool seg_path = c->lds_mode >= 1 && ng == 1 && lds_bytes <= kMaxDynLds;
+      const bool seg_path = var >= 1 && ng == 1 && lds_bytes <= kMaxDynLds;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (c->profiling) {
         e0 = c->get_event();
@@ -1382,6 +1392,20 @@ int cq_set_lds_digestion(cq_ctx_t* c, int mode) {
     if (mode < 0 || mode > 2) throw std::invalid_argument("digestion mode must be 0, 1 or 2");
     c->lds_mode = mode;
   });
+}
+
+int cq_set_class_digestion(cq_ctx_t* c, int k, int mode) {
+  return guarded(c, [&] {
+    if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");
+    if (mode < -1 || mode > 2) throw std::invalid_argument("digestion mode must be -1, 0, 1 or 2");
+    c->class_variant[k] = mode;
+  });
+}
+
+int cq_get_class_digestion(const cq_ctx_t* c, int k, int* mode) {
+  if (!c || !mode || k < 0 || k >= kNumClassPairs) return CQ_ERR_INVALID;
+  *mode = c->class_variant[k];
+  return CQ_OK;
 }
 
 // Number of streams the class-pair kernels are spread over (1 serialises them, for

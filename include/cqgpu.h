@@ -113,6 +113,13 @@ int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld);
  * an FP64 atomic.  Results agree to rounding in every mode. */
 int cq_set_lds_digestion(cq_ctx_t* ctx, int mode);
 
+/* Digestion variant of one class pair k (0 .. cq_num_class_pairs()-1): 0, 1 or 2 as in
+ * cq_set_lds_digestion (capped by the context's mode), -1 to follow the context's mode.  Lets
+ * a tuner pick the fastest variant per class pair from cq_get_kernel_times; results agree to
+ * rounding in every variant. */
+int cq_set_class_digestion(cq_ctx_t* ctx, int k, int mode);
+int cq_get_class_digestion(const cq_ctx_t* ctx, int k, int* mode);
+
 /* Number of HIP streams (1..4, default 4) the class-pair kernels of one contraction are spread
  * over; 1 runs them one after another (for per-kernel timing). */
 int cq_set_launch_streams(cq_ctx_t* ctx, int n);

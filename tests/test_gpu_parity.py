@@ -212,6 +212,33 @@ def test_every_digestion_mode_matches_oracle(lds):
     assert sum(ao.quartet_counts().values()) == n
 
 
+def test_per_class_digestion_variants_match_oracle():
+    """Class pairs on different digestion variants in one call (task tables 0 and 1 side by
+    side) and the autotuner's choice: J and K against the oracle ((H2O)8 / cc-pVDZ)."""
+    from chronusq_public_amd import _lib
+    import torch
+    L = _lib.load()
+    mol = M.water_cluster(8, seed=11, box_ang=M.scaled_box(8))
+    bs, ao, ob = _setup(mol, "cc-pvdz")
+    X = _sym_random(bs.nbasis, 7)
+    Qg = ao.computeSchwartz()
+    (Jo, Ko), n = _oracle_jk(ob, Qg, [X, X], [0, 1], [1.0, -0.5], 1e-12)
+    nc = L.cq_num_class_pairs()
+    for k in range(nc):
+        _lib.check(L.cq_set_class_digestion(ao._h, k, (2, 0, 1)[k % 3]), ao._h)
+    assert ao.class_digestion() == [(2, 0, 1)[k % 3] for k in range(nc)]
+    J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+    assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL
+    assert sum(ao.quartet_counts().values()) == n
+    choice = ao.autotune(torch.from_numpy(np.ascontiguousarray(X)).cuda(), reps=1)
+    assert len(choice) == nc and all(v in (0, 1, 2) for v, _ in choice.values())
+    assert ao.class_digestion() == [v for v, _ in choice.values()]
+    J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+    assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL
+    with pytest.raises(_lib.CQError):
+        _lib.check(L.cq_set_class_digestion(ao._h, nc, 0), ao._h)
+
+
 def test_mixed_precision_window_matches_fp64():
     """Mixed mode: quartets whose bound lies in [tau, 1e-5) get FP32 integrals.  On a
     difference-density-sized X ((H2O)4 / cc-pVDZ, |X| ~ 1e-3) most quartets fall in the FP32
