@@ -152,9 +152,10 @@ def parse_deck_text(text: str) -> Deck:
     if "FIELD" in scf and any(abs(x) > 0 for x in scf["FIELD"]):
         raise DeckError("static fields are not supported by this SCF driver")
     guess = scf.get("GUESS", "SAD").upper()
-    # LEARNED (the network guess of learned_guess.py) has no reference counterpart
-    if guess not in ("SAD", "CORE", "LEARNED"):
-        raise DeckError(f"guess {guess} is not supported (SAD, CORE or LEARNED)")
+    # LEARNED (the network guess of learned_guess.py) has no reference counterpart; READ
+    # takes the densities of the restart file <input basename>.bin
+    if guess not in ("SAD", "CORE", "READ", "LEARNED"):
+        raise DeckError(f"guess {guess} is not supported (SAD, CORE, READ or LEARNED)")
     if mol.natom == 1:
         guess = "CORE"           # the reference defaults to CORE for one atom (parseQM.py:84-87)
     alg = v.get("INTS", {}).get("ALG", "DIRECT").upper()

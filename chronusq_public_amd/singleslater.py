@@ -96,7 +96,8 @@ class SingleSlater:
                  incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True,
                  precision: str = "fp64", mixed_threshold: float = 1e-5,
                  pseudo_diag: bool = True, pdiag_switch: float = 1e-1,
-                 guess_model: Optional[str] = None, judge_guess: bool = True):
+                 guess_model: Optional[str] = None, judge_guess: bool = True,
+                 restart_file: Optional[str] = None):
         import torch
         self.torch = torch
         self.mol = mol
@@ -114,6 +115,9 @@ class SingleSlater:
         # the SAD density competes with it on energy
         self.guess_model, self.judge_guess = guess_model, judge_guess
         self.guess_info: Dict[str, object] = {}
+        # restart file in the reference's layout (restart.py): the READ guess reads its
+        # densities; SCF3 writes the converged ones into it
+        self.restart_file = restart_file
         self.ene_tol, self.den_tol, self.max_iter = ene_tol, den_tol, max_iter
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         t0 = time.perf_counter()
@@ -321,6 +325,17 @@ class SingleSlater:
         self._first_PT = (P0, PT0, PTz0)
         return P0, Mz0
 
+    def read_guess(self):
+        """Densities of a restart file (/SCF/DensityScalar, and DensityMz for UHF); the SCF's
+        first iteration forms their Fock matrix (singleslater_guess.h:320-329)."""
+        from . import restart
+        if not self.restart_file:
+            raise ValueError("the READ guess needs a restart file")
+        P, Mz = restart.read_densities(self.restart_file, self.N, open_shell=self.ref != "RHF")
+        t = lambda M: self.torch.from_numpy(np.ascontiguousarray(M)).to(self.dev)
+        self.guess_info = {"used": "READ", "file": self.restart_file}
+        return t(P), (t(Mz) if Mz is not None else None)
+
     # ------------------------------------------------------------------ SCF3
     def SCF3(self, P0=None, Mz0=None, reference_energy: Optional[float] = None) -> SCFResult:
         torch = self.torch
@@ -334,6 +349,8 @@ class SingleSlater:
             elif self.guess == "CORE":
                 P, Mz = self.core_guess()
                 self.guess_info = {"used": "CORE"}
+            elif self.guess == "READ":
+                P, Mz = self.read_guess()
             else:
                 raise ValueError(f"unknown guess {self.guess}")
         else:
@@ -453,6 +470,10 @@ class SingleSlater:
             raise SCFNotConverged("SCF Failed to converge within MAXITER iterations")
         res.energy = res.energies[-1]
         res.P, res.Mz = P, Mz
+        if self.restart_file:
+            from . import restart
+            restart.write_densities(self.restart_file, P.cpu().numpy(),
+                                    Mz.cpu().numpy() if Mz is not None else None)
         return res
 
 

@@ -77,3 +77,34 @@ def test_xyz_round_trip(tmp_path):
     back = read_xyz(str(p))
     assert list(back.Z) == list(mol.Z)
     assert np.abs(back.coords_bohr - mol.coords_bohr).max() < 1e-9
+
+
+def test_restart_file_round_trip_in_the_reference_layout(tmp_path):
+    """/SCF/DensityScalar and /SCF/DensityMz, N x N doubles holding the column-major buffer
+    (the dataset's row-major [i][j] is M(j, i)); shape and missing-data errors."""
+    import numpy as np
+    from chronusq_public_amd import restart as R
+    rng = np.random.default_rng(0)
+    A, Bm = rng.normal(size=(6, 6)), rng.normal(size=(6, 6))
+    path = str(tmp_path / "x.bin")
+    R.write_densities(path, A, Bm)
+    P, Mz = R.read_densities(path, 6, open_shell=True)
+    assert np.array_equal(P, A) and np.array_equal(Mz, Bm)
+    R.write_densities(path, 2 * A)                 # overwrite in place
+    assert np.array_equal(R.read_densities(path, 6)[0], 2 * A)
+    # raw dataset order: read the file's buffer without the transpose
+    L = R._lib()
+    f = L.H5Fopen(path.encode(), R.H5F_ACC_RDONLY, 0)
+    d = L.H5Dopen2(f, b"/SCF/DensityScalar", 0)
+    raw = np.empty((6, 6))
+    assert L.H5Dread(d, L.native_double, 0, 0, 0, raw.ctypes.data) >= 0
+    L.H5Dclose(d)
+    L.H5Fclose(f)
+    assert np.array_equal(raw, 2 * A.T)
+    with pytest.raises(R.RestartError):
+        R.read_densities(path, 7)
+    with pytest.raises(R.RestartError):
+        R.read_densities(str(tmp_path / "missing.bin"), 6)
+    R.write_densities(str(tmp_path / "closed.bin"), A)
+    with pytest.raises(R.RestartError):
+        R.read_densities(str(tmp_path / "closed.bin"), 6, open_shell=True)
