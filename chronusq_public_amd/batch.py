@@ -296,6 +296,8 @@ class BatchSCF:
             E_old = E
             out = None
             tp = self._tock("other", tp, sync=True)
+            # the batched eigensolve costs little per molecule: the rotations start only once
+            # every active molecule's energy moves by less than pdiag_switch
             if it > 0 and np.all(np.abs(dE[active]) < self.pdiag_switch):
                 out = self._pseudo_diag(F, C)
             if out is None:

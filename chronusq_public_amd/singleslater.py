@@ -87,6 +87,7 @@ class SCFResult:
     setup_parts: Dict[str, float] = field(default_factory=dict)
     guess_s: float = 0.0
     guess_info: Dict[str, object] = field(default_factory=dict)
+    phase_times: List[Dict[str, float]] = field(default_factory=list)   # with profile: per iteration
 
 
 class SingleSlater:
@@ -95,7 +96,7 @@ class SingleSlater:
                  max_iter: int = 256, device=None, aoints: Optional[AOIntegrals] = None,
                  incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True,
                  precision: str = "fp64", mixed_threshold: float = 1e-5,
-                 pseudo_diag: bool = True, pdiag_switch: float = 1e-1,
+                 pseudo_diag: bool = True, pdiag_switch: float = 1.0, pdiag_first: bool = True,
                  guess_model: Optional[str] = None, judge_guess: bool = True,
                  restart_file: Optional[str] = None):
         import torch
@@ -126,9 +127,13 @@ class SingleSlater:
         # mixed: FP32 integrals for quartets whose bound (inside the incremental build:
         # Schwarz x Schwarz x max|dD|) is below mixed_threshold; full FP64 rebuilds stay FP64
         self.precision, self.mixed_threshold = precision, mixed_threshold
-        # RHF density step: the eigensolve while |dE| > pdiag_switch, then first-order
-        # orbital rotations (pseudo-diagonalisation, GEMMs only)
+        # RHF density step: first-order orbital rotations of the previous (or the guess's)
+        # orbitals once |dE| < pdiag_switch (pseudo-diagonalisation, GEMMs only), the eigensolve
+        # before that and whenever an occupied-virtual gap closes
         self.pseudo_diag, self.pdiag_switch = pseudo_diag, pdiag_switch
+        # pdiag_first: the first two iterations (whose energy change is not known yet) already
+        # rotate the guess's orbitals
+        self.pdiag_first = pdiag_first
         t1 = time.perf_counter()
         S, T, V = self.ao.computeAOOneE(mol, self.dev)
         self.S, self.H = S, T + V
@@ -275,9 +280,12 @@ class SingleSlater:
     def _guess_from_fock(self, PT):
         F = self.H + 0.5 * PT
         na, nb = self.nocc
-        Pa, _ = self._diag_density(F, na)
         if self.ref == "RHF":
-            return 2.0 * Pa, None
+            Pa, C = self._diag_density(F, na, with_vectors=True)
+            P = 2.0 * Pa
+            self._guess_C = (P, C)     # orbitals of the starting density (pseudo-diagonalisation)
+            return P, None
+        Pa, _ = self._diag_density(F, na)
         Pb, _ = self._diag_density(F, nb)
         return Pa + Pb, Pa - Pb
 
@@ -314,7 +322,9 @@ class SingleSlater:
         self.guess_info = {"used": "LEARNED"}
         if not self.judge_guess:
             return P0_L
+        C_L = self._guess_C
         P0_S = self._guess_from_fock(self.formPT(self._atomic_densities())[0])
+        C_S = self._guess_C
         cand = []
         for name, (P0, Mz0) in (("LEARNED", P0_L), ("SAD", P0_S)):
             PT0, PTz0 = self.formPT(P0, Mz0 if uhf else None)
@@ -322,6 +332,7 @@ class SingleSlater:
         self.guess_info.update({"E0_learned": cand[0][0], "E0_sad": cand[1][0]})
         E0, name, P0, Mz0, PT0, PTz0 = min(cand, key=lambda c: c[0])
         self.guess_info["used"] = name
+        self._guess_C = C_L if name == "LEARNED" else C_S
         self._first_PT = (P0, PT0, PTz0)
         return P0, Mz0
 
@@ -337,10 +348,11 @@ class SingleSlater:
         return t(P), (t(Mz) if Mz is not None else None)
 
     # ------------------------------------------------------------------ SCF3
-    def SCF3(self, P0=None, Mz0=None, reference_energy: Optional[float] = None) -> SCFResult:
+    def SCF3(self, P0=None, Mz0=None, reference_energy: Optional[float] = None, profile: bool = False) -> SCFResult:
         torch = self.torch
         t_start = time.perf_counter()
         self._first_PT = None
+        self._guess_C = None
         if P0 is None:
             if self.guess == "SAD":
                 P, Mz = self.sad_guess()
@@ -357,6 +369,7 @@ class SingleSlater:
             self.guess_info = {"used": "INPUT"}
             P, Mz = P0, Mz0
         first_PT, self._first_PT = self._first_PT, None
+        guess_C, self._guess_C = self._guess_C, None
         torch.cuda.synchronize()
         guess_s = time.perf_counter() - t_start
         uhf = self.ref != "RHF"
@@ -367,7 +380,8 @@ class SingleSlater:
                         setup_parts=dict(self.setup_parts), guess_info=dict(self.guess_info))
         E_old = 0.0
         last_dE = float("inf")
-        C_orb = None    # orthonormal-basis orbitals of the last density step (RHF)
+        # orthonormal-basis orbitals of the last density step (RHF); the guess's own to start
+        C_orb = guess_C[1] if (guess_C is not None and guess_C[0] is P) else None
         Fo_prev = None
         P_prev = PT_prev = None
         B_hist = None
@@ -394,6 +408,10 @@ class SingleSlater:
             torch.cuda.synchronize()
             res.gd_times.append(time.perf_counter() - tg)
             E = self._energy(P, PT, Mz, PTz)
+            if profile:
+                torch.cuda.synchronize()
+                t_ph = time.perf_counter()
+                ph = {"gd": t_ph - tg}
             if uhf:
                 Fa = self.H + 0.5 * (PT + PTz)
                 Fb = self.H + 0.5 * (PT - PTz)
@@ -436,6 +454,10 @@ class SingleSlater:
                     c = c[:n]
                     if int(info) == 0 and bool(torch.isfinite(c).all()):
                         Fs = tuple(sum(c[i] * hist_F[i][k] for i in range(n)) for k in range(len(Fs)))
+            if profile:
+                torch.cuda.synchronize()
+                ph["energy_diis"] = time.perf_counter() - t_ph
+                t_ph = time.perf_counter()
             if damp > 0.0 and Fo_prev is not None:
                 Fs = tuple((1 - damp) * f + damp * fp for f, fp in zip(Fs, Fo_prev))
             Fo_prev = Fs
@@ -445,12 +467,17 @@ class SingleSlater:
                 Pn, Mzn = Pa_n + Pb_n, Pa_n - Pb_n
             else:
                 out = None
-                if self.pseudo_diag and C_orb is not None and abs(last_dE) < self.pdiag_switch:
+                if self.pseudo_diag and C_orb is not None and (abs(last_dE) < self.pdiag_switch or
+                                                               (it <= 1 and self.pdiag_first)):
                     out = self._pseudo_diag_density(Fs[0], na, C_orb)
                 if out is None:
                     out = self._diag_density(Fs[0], na, with_vectors=True)
                 Pa_n, C_orb = out
                 Pn, Mzn = 2.0 * Pa_n, None
+            if profile:
+                torch.cuda.synchronize()
+                ph["density"] = time.perf_counter() - t_ph
+                t_ph = time.perf_counter()
             dP = float(torch.linalg.norm(Pn - P))
             if uhf:
                 dP = max(dP, float(torch.linalg.norm(Mzn - Mz)))
@@ -462,6 +489,9 @@ class SingleSlater:
             P, Mz = Pn, Mzn
             res.energies.append(E)
             res.times.append(time.perf_counter() - t_start)
+            if profile:
+                ph["rest"] = time.perf_counter() - t_ph
+                res.phase_times.append(ph)
             res.iterations = it + 1
             if it > 0 and abs(dE) < self.ene_tol and dP < self.den_tol:
                 res.converged = True
