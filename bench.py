@@ -122,6 +122,33 @@ def cpu_baseline(bs, D, tau, nthreads, ncalls=3):
                       f"of {ncalls} calls, {dt:.2f} s per call"}
 
 
+def run_autotuned(bs, args, dD, dJ, dK, stream, nquart, e0, e1):
+    """G[D] with per-class digestion variants chosen by AOIntegrals.autotune, timed over the
+    same steps as the production configuration."""
+    import torch
+    from chronusq_public_amd.aointegrals import AOIntegrals, COULOMB, EXCHANGE
+    ao_t = AOIntegrals(bs)
+    ao_t.thresholdSchwartz = args.tau
+    choice = ao_t.autotune(dD, reps=2)
+    for _ in range(args.warmup):
+        dJ.zero_()
+        dK.zero_()
+        ao_t.newTwoEContractDirect([dD, dD], [dJ, dK], [COULOMB, EXCHANGE], [1.0, -0.5], stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        dJ.zero_()
+        dK.zero_()
+        ao_t.newTwoEContractDirect([dD, dD], [dJ, dK], [COULOMB, EXCHANGE], [1.0, -0.5], stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_tuned = e0.elapsed_time(e1) / args.steps
+    ao_t.close()
+    return {"ms_per_step": ms_tuned, "quartets_per_s": nquart / (ms_tuned * 1e-3),
+            "variants": {k: v for k, (v, _) in choice.items()},
+            "variant_ms": {k: t for k, (_, t) in choice.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +158,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-molecule SCF")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the per-class digestion autotuner")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run three 256 MiB device-to-device copies (a plain copy kernel of "
                          "known size, the scale for HBM byte counters)")
@@ -283,26 +311,9 @@ def main():
     counts = ao.quartet_counts()
     nquart = sum(counts.values())
     # per-class digestion variants picked by the autotuner (timed once, then the same K steps)
-    ao_t = AOIntegrals(bs)
-    ao_t.thresholdSchwartz = args.tau
-    choice = ao_t.autotune(dD, reps=2)
-    for _ in range(args.warmup):
-        dJ.zero_()
-        dK.zero_()
-        ao_t.newTwoEContractDirect([dD, dD], [dJ, dK], [COULOMB, EXCHANGE], [1.0, -0.5], stream=stream.cuda_stream)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        dJ.zero_()
-        dK.zero_()
-        ao_t.newTwoEContractDirect([dD, dD], [dJ, dK], [COULOMB, EXCHANGE], [1.0, -0.5], stream=stream.cuda_stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_tuned = e0.elapsed_time(e1) / args.steps
-    autotuned = {"ms_per_step": ms_tuned, "quartets_per_s": nquart / (ms_tuned * 1e-3),
-                 "variants": {k: v for k, (v, _) in choice.items()},
-                 "variant_ms": {k: t for k, (_, t) in choice.items()}}
-    ao_t.close()
+    autotuned = None
+    if not args.no_autotune:
+        autotuned = run_autotuned(bs, args, dD, dJ, dK, stream, nquart, e0, e1)
     ws = (ctypes.c_double * (4 * ncp))()
     _lib.check(L.cq_get_work_stats(ao._h, ws, ncp), ao._h)
     per_class = []

@@ -22,9 +22,11 @@ def run_deck(path: str, out=sys.stdout, restart_file: str = None) -> dict:
     mol = deck.molecule
     restart_file = restart_file or os.path.splitext(path)[0] + ".bin"
     t0 = time.perf_counter()
+    # the reference's SCF always damps the orthonormal Fock by 0.1 while |dE| > 1e-6
+    # (include/singleslater.h:472-473)
     ss = SingleSlater(mol, deck.basis, ref=deck.reference, guess=deck.guess, ene_tol=deck.ene_tol,
                       den_tol=deck.den_tol, max_iter=deck.max_iter, ndiis=10 if deck.diis else 0,
-                      restart_file=restart_file)
+                      restart_file=restart_file, damp=0.1)
     res = ss.SCF3()
     wall = time.perf_counter() - t0
     print(f"  Molecule: {mol.natom} atoms, charge {mol.charge}, multiplicity {mol.mult}", file=out)

@@ -484,8 +484,12 @@ class SingleSlater:
             dE = E - E_old
             last_dE = dE
             E_old = E
+            # damping off once |dE| < 1e-6, back on if the energy moves by more again
+            # (singleslater_scfutils.h:262-280)
             if abs(dE) < 1e-6:
                 damp = 0.0
+            elif self.damp > 0.0 and damp <= 0.0:
+                damp = self.damp
             P, Mz = Pn, Mzn
             res.energies.append(E)
             res.times.append(time.perf_counter() - t_start)

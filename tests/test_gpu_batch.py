@@ -23,9 +23,11 @@ def test_batch_matches_single_molecule_scf():
     b.close()
     assert r.converged.all()
     for k, m in enumerate(mols):
-        s = SingleSlater(m, "6-31g*").SCF3()
+        # the batch's density-step schedule: eigensolve until |dE| < 0.1, then rotations
+        s = SingleSlater(m, "6-31g*", pdiag_switch=0.1, pdiag_first=False).SCF3()
         assert abs(r.energies[k] - s.energy) < 1e-9, (k, r.energies[k], s.energy)
         assert abs(int(r.iterations[k]) - s.iterations) <= 1
+        assert abs(r.energies[k] - SingleSlater(m, "6-31g*").SCF3().energy) < 1e-9
 
 
 def test_batch_reaches_golden_water_energy():

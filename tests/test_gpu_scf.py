@@ -106,16 +106,20 @@ def test_mixed_precision_incremental_scf_energy():
 
 
 def test_pseudo_diagonalisation_matches_eigensolve():
-    """The RHF density step that rotates the previous orbitals (after |dE| < 0.1) converges
-    to the same energy as diagonalising every iteration ((H2O)4 / cc-pVDZ), in about as many
-    iterations."""
+    """The RHF density step that rotates the previous orbitals converges to the same energy as
+    diagonalising every iteration ((H2O)4 / cc-pVDZ): from the guess's orbitals on (the
+    default; a few more iterations, each far cheaper than an eigensolve) and after |dE| < 0.1
+    (about as many iterations)."""
     from chronusq_public_amd.singleslater import SingleSlater
     mol = M.water_cluster(4, seed=3, box_ang=M.scaled_box(4))
     ref = SingleSlater(mol, "cc-pvdz", tau=1e-12, pseudo_diag=False).SCF3()
-    pd = SingleSlater(mol, "cc-pvdz", tau=1e-12, pseudo_diag=True).SCF3()
-    assert pd.converged
-    assert abs(pd.energy - ref.energy) < 1e-9
-    assert pd.iterations <= ref.iterations + 2
+    pd = SingleSlater(mol, "cc-pvdz", tau=1e-12).SCF3()
+    late = SingleSlater(mol, "cc-pvdz", tau=1e-12, pdiag_switch=0.1, pdiag_first=False).SCF3()
+    for r in (pd, late):
+        assert r.converged
+        assert abs(r.energy - ref.energy) < 1e-9
+    assert pd.iterations <= ref.iterations + 4
+    assert late.iterations <= ref.iterations + 2
 
 
 def test_input_deck_runs_to_golden_energy(tmp_path):
