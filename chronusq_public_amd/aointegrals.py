@@ -89,12 +89,14 @@ class AOIntegrals:
             raise ValueError(f"unknown precision mode {mode!r}")
         check(_lib.load().cq_set_precision_mode(self._h, code, float(threshold)), self._h)
 
-    def autotune(self, X, reps: int = 3, variants: Sequence[int] = (0, 1, 2)) -> dict:
+    def autotune(self, X, reps: int = 3, variants: Sequence[int] = (0, 1, 2), balance_streams: bool = True) -> dict:
         """Pick, per class pair, the fastest J/K digestion variant for the RHF contraction of
         the device density X (0: FP64 atomics into HBM, 1: ket-keyed LDS segments, 2: also
         bra-keyed segments where the class pair allows them).  Every variant runs `reps`
         times with the class kernels serialised and timed by HIP events
         (cq_get_kernel_times); the fastest is set per class pair (cq_set_class_digestion).
+        With balance_streams the class pairs are then placed on the four streams longest
+        first, each on the stream with the least kernel time so far (cq_set_class_stream).
         Returns {class name: (chosen variant, {variant: mean ms})}.  No reference
         counterpart."""
         import torch
@@ -121,10 +123,18 @@ class AOIntegrals:
             check(L.cq_set_launch_streams(self._h, 4), self._h)
         names = list(self.quartet_counts().keys())
         out = {}
+        chosen_ms = []
         for k in range(nc):
             best = min(times, key=lambda v: times[v][k])
             check(L.cq_set_class_digestion(self._h, k, best), self._h)
             out[names[k]] = (best, {v: times[v][k] for v in times})
+            chosen_ms.append(times[best][k])
+        load = [0.0] * 4
+        for k in sorted(range(nc), key=lambda k: -chosen_ms[k]):
+            s = min(range(4), key=lambda i: load[i]) if balance_streams else -1
+            check(L.cq_set_class_stream(self._h, k, s), self._h)
+            if s >= 0:
+                load[s] += chosen_ms[k]
         return out
 
     def class_digestion(self) -> List[int]:

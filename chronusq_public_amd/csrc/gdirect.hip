@@ -586,6 +586,8 @@ struct cq_ctx {
   int lds_mode = 2;   // 0 HBM atomics, 1 ket-keyed LDS segments, 2 also bra-keyed segments
   // per class pair: the digestion variant (0..2 as lds_mode, capped by it), -1 = lds_mode
   std::vector<int> class_variant = std::vector<int>(CQ_NUM_CLASS_PAIRS, -1);
+  // per class pair: the auxiliary stream its kernels go to (-1: round robin in launch order)
+  std::vector<int> class_stream = std::vector<int>(CQ_NUM_CLASS_PAIRS, -1);
   DevBuf<unsigned long long> d_counts, d_qcounts, d_nglob;
   DevBuf<unsigned char> d_pair_nblk, d_pair_nublk;
   std::string err;
@@ -1093,7 +1095,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const int tb = (var >= 2 && nslot == 1) ? 1 : 0;
       const CPMeta& m = c->cps[tb][k];
       if (m.nbra == 0 || m.nket == 0) continue;
-      hipStream_t sk = c->aux[launched++ % c->n_streams];
+      hipStream_t sk = c->class_stream[k] >= 0 ? c->aux[c->class_stream[k] % c->n_streams]
+                                               : c->aux[launched++ % c->n_streams];
       if (s0 == 0) {
         const int nuni = m.flip ? m.nbra : m.nket;
         hipLaunchKernelGGL(screen_kernel, dim3(grid_for(nuni, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p,
@@ -1399,6 +1402,14 @@ int cq_set_class_digestion(cq_ctx_t* c, int k, int mode) {
     if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");
     if (mode < -1 || mode > 2) throw std::invalid_argument("digestion mode must be -1, 0, 1 or 2");
     c->class_variant[k] = mode;
+  });
+}
+
+int cq_set_class_stream(cq_ctx_t* c, int k, int stream) {
+  return guarded(c, [&] {
+    if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");
+    if (stream < -1 || stream >= cq_ctx::kAux) throw std::invalid_argument("stream index must be -1 .. 3");
+    c->class_stream[k] = stream;
   });
 }
 
