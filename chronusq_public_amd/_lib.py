@@ -22,7 +22,7 @@ CQ_PREC_MIXED = 1
 
 # every symbol include/cqgpu.h declares
 EXPORTED = [
-    "cq_ctx_create", "cq_ctx_destroy", "cq_last_error", "cq_set_schwarz_threshold",
+    "cq_ctx_create", "cq_ctx_create_batch", "cq_ctx_destroy", "cq_last_error", "cq_set_schwarz_threshold",
     "cq_get_schwarz_threshold", "cq_set_precision_mode", "cq_get_dims", "cq_get_schwarz",
     "cq_contract_direct", "cq_contract_direct_device", "cq_get_quartet_counts",
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
@@ -59,6 +59,7 @@ def load():
     P = ctypes.POINTER
     dp, ip, vp = P(ctypes.c_double), P(ctypes.c_int), ctypes.c_void_p
     L.cq_ctx_create.argtypes = [P(vp), ctypes.c_int, ip, ip, ip, dp, dp, dp]
+    L.cq_ctx_create_batch.argtypes = [P(vp), ctypes.c_int, ip, ip, ip, dp, dp, dp, ip]
     L.cq_ctx_destroy.argtypes = [vp]
     L.cq_last_error.argtypes = [vp]
     L.cq_last_error.restype = ctypes.c_char_p

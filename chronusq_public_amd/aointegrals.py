@@ -58,6 +58,32 @@ class AOIntegrals:
         check(L.cq_get_dims(self._h, *[ctypes.byref(d) for d in dims]), self._h)
         self.nShell, self.nBasis, self.nCart, self.nShellPair, self.nPrimPair = (d.value for d in dims)
 
+    @classmethod
+    def batch(cls, bases: Sequence) -> "AOIntegrals":
+        """One context for several molecules (cq_ctx_create_batch): their shells concatenated
+        in order, shell pairs and quartets formed only within a molecule.  A contraction over
+        the block-diagonal density of all molecules (functions in the same order) gives every
+        molecule's G[D] in its diagonal block.  No reference counterpart."""
+        self = cls.__new__(cls)
+        self.basis = list(bases)
+        self.integralAlgorithm = DIRECT
+        parts = [b.flat() for b in self.basis]
+        self._flat = tuple(np.ascontiguousarray(np.concatenate([p[i] for p in parts]),
+                                                np.int32 if i < 3 else np.float64) for i in range(6))
+        mol = np.ascontiguousarray(np.concatenate([np.full(len(p[0]), k, dtype=np.int32)
+                                                   for k, p in enumerate(parts)]))
+        self._mol = mol
+        f = self._flat
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        check(L.cq_ctx_create_batch(ctypes.byref(h), len(f[0]), _ip(f[0]), _ip(f[1]), _ip(f[2]), _dp(f[3]),
+                                    _dp(f[4]), _dp(f[5]), _ip(mol)), None)
+        self._h = h
+        dims = [ctypes.c_int() for _ in range(5)]
+        check(L.cq_get_dims(self._h, *[ctypes.byref(d) for d in dims]), self._h)
+        self.nShell, self.nBasis, self.nCart, self.nShellPair, self.nPrimPair = (d.value for d in dims)
+        return self
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().cq_ctx_destroy(self._h)

@@ -1,10 +1,13 @@
 """Batched RHF SCF of many uneven molecules on one GPU.  No reference counterpart (the
 reference runs one molecule per job, src/python/src/standardJobs.py:55-99).
 
-Every molecule keeps its own library context (shell pairs, Schwarz bounds, screening) and its
-own G[D] call, enqueued back to back on one stream with no host synchronisation between
-them; everything else of the SCF runs once for the whole batch on matrices padded to the
-largest basis:
+G[D]: the molecules are split into groups of at most group_functions basis functions; each
+group is one library context over all its molecules' shells (cq_ctx_create_batch: shell pairs
+and quartets only within a molecule), and one contraction of the group's block-diagonal
+density -- one screening and one ERI launch per class pair for every molecule of the group --
+gives each molecule's G[D] in its diagonal block (merged=False: one context and one call per
+molecule, spread over four streams).  Everything else of the SCF runs once for the whole
+batch on matrices padded to the largest basis:
 
 * overlap padded with the identity, core Hamiltonian and Fock with zeros, so the padding
   block decouples; the orthonormal Fock's padding diagonal is set to 1e3 Eh, above every
@@ -33,7 +36,7 @@ import numpy as np
 
 from . import _lib
 from . import basis as B
-from .aointegrals import AOIntegrals
+from .aointegrals import AOIntegrals, COULOMB, EXCHANGE
 from .molecule import Molecule, alkane, scaled_box, water_cluster
 from .singleslater import atomic_density
 
@@ -57,7 +60,8 @@ class BatchSCF:
                  ene_tol: float = 1e-10, den_tol: float = 1e-8, max_iter: int = 128, device=None,
                  pdiag_switch: float = 1e-1, precision: str = "fp64", mixed_threshold: float = 1e-5,
                  incremental: bool = False, rebuild_every: int = 8, launch_streams: int = 4,
-                 streams: int = 4, setup_threads: int = 8, profile: bool = False):
+                 streams: int = 4, setup_threads: int = 8, profile: bool = False, merged: bool = True,
+                 group_functions: int = 4096):
         import torch
         self.torch = torch
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
@@ -119,8 +123,88 @@ class BatchSCF:
         self.pad_level = torch.diag_embed(pad * PAD_LEVEL)
         self.occ = (torch.arange(Nm, device=self.dev)[None, :] < torch.from_numpy(self.nocc).to(self.dev)[:, None])
         self.X = self._lowdin(self.S)
+        # merged: one batch context (cq_ctx_create_batch) per group of molecules of at most
+        # group_functions functions; each G[D] of the group is one contraction over the
+        # block-diagonal density, one launch per class pair for all its molecules
+        self.merged = merged
+        self.groups = []
+        if merged:
+            self._make_groups(group_functions)
+            for ao in self.ao:
+                ao.close()            # the per-molecule contexts served the one-electron integrals
+            self.ao = []
         torch.cuda.synchronize()
         self.setup_s = time.perf_counter() - t0
+
+    def _make_groups(self, max_functions: int):
+        torch = self.torch
+        members, cur, tot = [], [], 0
+        for k in range(len(self.mols)):
+            if cur and tot + int(self.N[k]) > max_functions:
+                members.append(cur)
+                cur, tot = [], 0
+            cur.append(k)
+            tot += int(self.N[k])
+        if cur:
+            members.append(cur)
+        Nm = self.Nmax
+        for mem in members:
+            ao = AOIntegrals.batch([self.bases[k] for k in mem])
+            ao.thresholdSchwartz = self.tau
+            n = ao.nBasis
+            dense, pad = [], []
+            off = 0
+            for k in mem:
+                nk = int(self.N[k])
+                r = np.arange(nk)
+                dense.append(((off + r)[:, None] * n + (off + r)[None, :]).ravel())
+                pad.append((k * Nm * Nm + r[:, None] * Nm + r[None, :]).ravel())
+                off += nk
+            f64 = dict(dtype=torch.float64, device=self.dev)
+            self.groups.append(dict(
+                ao=ao, members=mem, n=n,
+                dense=torch.from_numpy(np.concatenate(dense)).to(self.dev),
+                pad=torch.from_numpy(np.concatenate(pad)).to(self.dev),
+                X=torch.zeros((n, n), **f64), J=torch.zeros((n, n), **f64), K=torch.zeros((n, n), **f64)))
+
+    def _formPT_merged(self, X, active: np.ndarray, mixed: bool):
+        """One contraction per group, the groups spread round-robin over the streams so that
+        their kernels overlap; each group writes its molecules' diagonal blocks of the result."""
+        torch = self.torch
+        act = torch.from_numpy(active).to(self.dev).double()[:, None, None]
+        Xm = (X * act).reshape(-1)          # converged molecules contribute nothing (screened out)
+        out = torch.zeros_like(X)
+        main = torch.cuda.current_stream(self.dev)
+        ready = torch.cuda.Event()
+        ready.record(main)
+        used = set()
+        for i, g in enumerate(self.groups):
+            if not active[g["members"]].any():
+                continue
+            st = self.streams[i % len(self.streams)]
+            if st not in used:
+                st.wait_event(ready)
+                used.add(st)
+            ao = g["ao"]
+            with torch.cuda.stream(st):
+                g["X"].zero_()
+                g["X"].view(-1)[g["dense"]] = Xm[g["pad"]]
+                g["J"].zero_()
+                g["K"].zero_()
+                if mixed:
+                    ao.setPrecision("mixed", self.mixed_threshold)
+                try:
+                    ao.newTwoEContractDirect([g["X"], g["X"]], [g["J"], g["K"]], [COULOMB, EXCHANGE], [1.0, -0.5],
+                                             stream=st.cuda_stream)
+                finally:
+                    if mixed:
+                        ao.setPrecision("fp64")
+                out.view(-1)[g["pad"]] = (g["J"] + g["K"]).view(-1)[g["dense"]]
+        for st in used:
+            done = torch.cuda.Event()
+            done.record(st)
+            main.wait_event(done)
+        return out
 
     # ---------------------------------------------------------------- pieces
     def _lowdin(self, S):
@@ -145,9 +229,11 @@ class BatchSCF:
         with dP the contraction is of dP (the incremental build)."""
         torch = self.torch
         L = _lib.load()
+        X = P if dP is None else dP
+        if self.merged:
+            return self._formPT_merged(X, active, mixed)
         J = torch.zeros_like(P)
         K = torch.zeros_like(P)
-        X = P if dP is None else dP
         main = torch.cuda.current_stream(self.dev)
         ready = torch.cuda.Event()
         ready.record(main)
@@ -330,6 +416,8 @@ class BatchSCF:
     def close(self):
         for ao in self.ao:
             ao.close()
+        for g in self.groups:
+            g["ao"].close()
 
 
 def c4_molecules(seed: int = 0, n_alkane: int = 128, n_water: int = 128) -> List[Molecule]:

@@ -75,7 +75,8 @@ __device__ __forceinline__ void wave_sync_lds() {
 __global__ void __launch_bounds__(WG) screen_kernel(
     const CPMeta* __restrict__ cps, int ncp, const int* __restrict__ item_start, int nitem,
     const int2* __restrict__ s12, const double* __restrict__ pairQ, const int* __restrict__ grp,
-    const int* __restrict__ cls_grp_off, const double* __restrict__ Nmax,
+    const int* __restrict__ cls_grp_off, const int* __restrict__ pair_mol, const int* __restrict__ cmg, int nmol,
+    const double* __restrict__ Nmax,
     const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, double tau_hi,
     const unsigned char* __restrict__ pair_nblk, const unsigned char* __restrict__ pair_nublk,
     uint2* __restrict__ tasks,
@@ -121,8 +122,11 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     wave_sync_lds();
     nb = 0;
   };
-  const int g_lo = cls_grp_off[sw_class], g_hi = cls_grp_off[sw_class + 1];
-  for (int g = g_lo; g < g_hi; ++g) {
+  // the swept pairs of the uniform pair's own molecule (one molecule: the whole class)
+  const int g_hi = cls_grp_off[sw_class + 1];
+  const int mK = pair_mol[K];
+  const int gm_lo = cmg[sw_class * (nmol + 1) + mK], gm_hi = cmg[sw_class * (nmol + 1) + mK + 1];
+  for (int g = gm_lo; g < gm_hi; ++g) {
   const int grp_end = (g + 1 < g_hi) ? grp[g + 1] : nsw;
   if (grp_end <= i_begin) continue;
   for (int i0 = max(grp[g], i_begin); i0 < grp_end; i0 += 64) {
@@ -574,6 +578,11 @@ struct cq_ctx {
   DevBuf<int2> d_s12;
   std::vector<int> grp, cls_grp_off;
   DevBuf<int> d_grp, d_cls_grp_off;
+  // several molecules in one context: shell pairs and quartets only within a molecule
+  int nmol = 1, lds_n = 0;            // lds_n: the largest molecule's Cartesian functions
+  std::vector<int> shell_mol;
+  std::vector<int> cmg;               // npc x (nmol + 1): first screening group of each molecule
+  DevBuf<int> d_pair_mol, d_cmg;
   int n_streams = 4;
   static constexpr int kAux = 4;
   hipStream_t aux[kAux] = {};
@@ -604,7 +613,7 @@ struct cq_ctx {
       if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
-    d_s12.release(); d_grp.release(); d_cls_grp_off.release();
+    d_s12.release(); d_grp.release(); d_cls_grp_off.release(); d_pair_mol.release(); d_cmg.release();
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -627,7 +636,7 @@ int persistent_grid() {
 }
 
 void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
-                   const double* coef, const double* origin) {
+                   const double* coef, const double* origin, const int* mol = nullptr) {
   HIPCHK(hipGetDevice(&c->device));
   HIPCHK(hipStreamCreate(&c->stream));
   for (int k = 0; k < cq_ctx::kAux; ++k) {
@@ -653,6 +662,21 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   }
   c->nsph = sph;
   c->ncart = cart;
+  // molecules: consecutive shells, ids 0, 1, ... in order (nullptr: one molecule)
+  c->shell_mol.assign(nshell, 0);
+  if (mol) {
+    if (nshell > 0 && mol[0] != 0) throw std::invalid_argument("molecule ids must start at 0");
+    for (int s = 0; s < nshell; ++s) {
+      if (s > 0 && mol[s] != mol[s - 1] && mol[s] != mol[s - 1] + 1)
+        throw std::invalid_argument("molecule ids must be consecutive and nondecreasing over the shells");
+      c->shell_mol[s] = mol[s];
+    }
+  }
+  c->nmol = nshell > 0 ? c->shell_mol[nshell - 1] + 1 : 1;
+  std::vector<int> mol_cart0(c->nmol + 1, cart);
+  for (int s = nshell - 1; s >= 0; --s) mol_cart0[c->shell_mol[s]] = c->shells[s].cart_off;
+  c->lds_n = 0;
+  for (int m = 0; m < c->nmol; ++m) c->lds_n = std::max(c->lds_n, mol_cart0[m + 1] - mol_cart0[m]);
   c->npc = 6;   // ss ps pp ds dp dd
   // solid-harmonic tables for l = 0..2
   c->tco_off.assign(5, 0);
@@ -670,11 +694,13 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   const double pi = 3.14159265358979323846;
   for (int s1 = 0; s1 < nshell; ++s1)
     for (int s2 = 0; s2 <= s1; ++s2) {
+      if (c->shell_mol[s2] != c->shell_mol[s1]) continue;   // pairs only within a molecule
       int a = s1, b = s2;
       if (l[s2] > l[s1]) std::swap(a, b);
       HostPair h;
       h.info.a = a; h.info.b = b; h.info.s1 = s1; h.info.s2 = s2;
       h.info.a0 = c->shells[a].cart_off; h.info.b0 = c->shells[b].cart_off;
+      h.info.loff = mol_cart0[c->shell_mol[s1]];
       h.cls = pair_class(l[a], l[b]);
       const double* A = origin + 3 * a;
       const double* B = origin + 3 * b;
@@ -827,6 +853,8 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   std::iota(order.begin(), order.end(), 0);
   for (int X = 0; X < c->npc; ++X)
     std::stable_sort(order.begin() + c->class_start[X], order.begin() + c->class_start[X + 1], [&](int x, int y) {
+      const int mx = c->shell_mol[hp[x].info.s1], my = c->shell_mol[hp[y].info.s1];
+      if (mx != my) return mx < my;
       const size_t nx = hp[x].prim.size(), ny = hp[y].prim.size();
       if (nx != ny) return nx > ny;
       return c->pairQ[x] > c->pairQ[y];
@@ -845,17 +873,29 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   // starts (class-local), closed by the class size
   c->cls_grp_off.assign(c->npc + 1, 0);
   c->grp.clear();
+  const int nm = c->nmol;
+  c->cmg.assign((size_t)c->npc * (nm + 1), 0);
+  std::vector<int> pair_mol(npair);
+  for (int k = 0; k < npair; ++k) pair_mol[k] = c->shell_mol[c->pairs[k].s1];
   for (int X = 0; X < c->npc; ++X) {
     c->cls_grp_off[X] = (int)c->grp.size();
+    int* cm = &c->cmg[(size_t)X * (nm + 1)];
+    int next_m = 0;   // molecules whose first group is not yet known
     for (int k = c->class_start[X]; k < c->class_start[X + 1]; ++k) {
       const int np_ = c->pairs[k].p1 - c->pairs[k].p0;
-      if (k == c->class_start[X] || np_ != c->pairs[k - 1].p1 - c->pairs[k - 1].p0)
+      const bool new_mol = k == c->class_start[X] || pair_mol[k] != pair_mol[k - 1];
+      if (new_mol || np_ != c->pairs[k - 1].p1 - c->pairs[k - 1].p0) {
+        while (new_mol && next_m <= pair_mol[k]) cm[next_m++] = (int)c->grp.size();
         c->grp.push_back(k - c->class_start[X]);
+      }
     }
+    while (next_m <= nm) cm[next_m++] = (int)c->grp.size();
   }
   c->cls_grp_off[c->npc] = (int)c->grp.size();
   c->d_grp.upload(c->grp, c->stream);
   c->d_cls_grp_off.upload(c->cls_grp_off, c->stream);
+  c->d_cmg.upload(c->cmg, c->stream);
+  c->d_pair_mol.upload(pair_mol, c->stream);
   {
     std::vector<int2> s12(npair);
     for (int k = 0; k < npair; ++k) s12[k] = make_int2(c->pairs[k].s1, c->pairs[k].s2);
@@ -912,16 +952,32 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     class_blocks(ck.ki, nblk, ksum, kmax);
     class_blocks(ck.bi, nublk, busum, bumax);
     class_blocks(ck.ki, nublk, kusum, kumax);
-    // task entries: every swept pair's blocks against every uniform pair (same class: bra >= ket)
+    // task entries: every swept pair's blocks against every uniform pair of its molecule
+    // (same class: bra >= ket)
+    std::vector<long long> bsum_m(nm, 0), ksum_m(nm, 0), nbra_m(nm, 0), nket_m(nm, 0);
+    for (int i = 0; i < m.nbra; ++i) {
+      bsum_m[pair_mol[m.bra_start + i]] += nblk[m.bra_start + i];
+      nbra_m[pair_mol[m.bra_start + i]] += 1;
+    }
+    for (int i = 0; i < m.nket; ++i) {
+      ksum_m[pair_mol[m.ket_start + i]] += nblk[m.ket_start + i];
+      nket_m[pair_mol[m.ket_start + i]] += 1;
+    }
     long long cap_ket = 0;
     if (m.same) {
-      for (int i = 0; i < m.nbra; ++i) cap_ket += (long long)nblk[m.bra_start + i] * (i + 1);
+      int first = 0;   // class-local index of the first pair of the current molecule
+      for (int i = 0; i < m.nbra; ++i) {
+        if (i > 0 && pair_mol[m.bra_start + i] != pair_mol[m.bra_start + i - 1]) first = i;
+        cap_ket += (long long)nblk[m.bra_start + i] * (i - first + 1);
+      }
     } else {
-      cap_ket = bsum * m.nket;
+      for (int q = 0; q < nm; ++q) cap_ket += bsum_m[q] * nket_m[q];
     }
-    const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->ncart + (size_t)ck.na * ck.nb) * sizeof(double);
+    const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
     const int flip = (!m.same && ck.jksegb && ldsb <= kMaxDynLds && 2LL * m.nbra <= m.nket) ? 1 : 0;
-    const long long cap_bra = flip ? ksum * m.nbra : 0;
+    long long cap_bra = 0;
+    if (flip)
+      for (int q = 0; q < nm; ++q) cap_bra += ksum_m[q] * nbra_m[q];
     m.task_off = off;
     off += std::max(cap_ket, cap_bra);
     for (int t = 0; t < 2; ++t) {
@@ -1074,6 +1130,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
     const int ng = std::min(MAX_SLOTS, nslot - s0);
     JKArgs jk{};
     jk.ncart = c->ncart;
+    jk.lds_n = c->lds_n;
     for (int g = 0; g < ng; ++g) {
       double* base = c->d_work.p + (size_t)(s0 + g) * 5 * nc2;
       jk.D[g] = base;
@@ -1101,7 +1158,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
         const int nuni = m.flip ? m.nbra : m.nket;
         hipLaunchKernelGGL(screen_kernel, dim3(grid_for(nuni, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p,
                            kNumClassPairs, c->d_item_start[tb].p, nuni, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
-                           c->d_cls_grp_off.p, c->d_Nmax.p, c->d_nglob.p, ns, lo, hi, c->d_pair_nblk.p,
+                           c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p, c->nmol, c->d_Nmax.p, c->d_nglob.p, ns,
+                           lo, hi, c->d_pair_nblk.p,
                            c->d_pair_nublk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p, c->d_segs.p, c->d_nseg.p, k);
         HIPCHK(hipGetLastError());
       }
@@ -1111,7 +1169,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const int grid = (int)std::min<long long>(persistent_grid(), (items + WG - 1) / WG);
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
       // one density slot and LDS room for the K columns: segment kernel (LDS accumulation)
-      const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->ncart + (size_t)ck.nc * ck.nd) * sizeof(double);
+      const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->lds_n + (size_t)ck.nc * ck.nd) * sizeof(double);
       const bool seg_path = var >= 1 && ng == 1 && lds_bytes <= kMaxDynLds;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (c->profiling) {
@@ -1121,7 +1179,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       }
       if (m.flip) {
         // bra-keyed segments (table 1 only flips where the LDS rows fit)
-        const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->ncart + (size_t)ck.na * ck.nb) * sizeof(double);
+        const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
         const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
         hipLaunchKernelGGL(fp32 ? ck.jksegb32 : ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
                            c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
@@ -1197,8 +1255,8 @@ thread_local std::string g_create_error;
 // =======================================================================================
 extern "C" {
 
-int cq_ctx_create(cq_ctx_t** out, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
-                  const double* coef, const double* origin) {
+int cq_ctx_create_batch(cq_ctx_t** out, int nshell, const int* l, const int* pure, const int* nprim,
+                        const double* alpha, const double* coef, const double* origin, const int* molecule) {
   if (!out) return CQ_ERR_INVALID;
   *out = nullptr;
   if (nshell < 1 || !l || !pure || !nprim || !alpha || !coef || !origin) {
@@ -1208,7 +1266,7 @@ int cq_ctx_create(cq_ctx_t** out, int nshell, const int* l, const int* pure, con
   cq_ctx* c = new (std::nothrow) cq_ctx();
   if (!c) return CQ_ERR_NOMEM;
   try {
-    build_context(c, nshell, l, pure, nprim, alpha, coef, origin);
+    build_context(c, nshell, l, pure, nprim, alpha, coef, origin, molecule);
   } catch (const std::invalid_argument& e) {
     g_create_error = e.what();
     delete c;
@@ -1227,6 +1285,11 @@ int cq_ctx_destroy(cq_ctx_t* c) {
   (void)hipStreamSynchronize(c->stream);
   delete c;
   return CQ_OK;
+}
+
+int cq_ctx_create(cq_ctx_t** out, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                  const double* coef, const double* origin) {
+  return cq_ctx_create_batch(out, nshell, l, pure, nprim, alpha, coef, origin, nullptr);
 }
 
 const char* cq_last_error(const cq_ctx_t* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
@@ -1351,6 +1414,9 @@ int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, 
   return guarded(c, [&] {
     if (natom < 0 || (natom > 0 && (!Z || !xyz)) || !S || !T || !V || ld < c->nsph)
       throw std::invalid_argument("invalid one-electron arguments");
+    if (c->nmol > 1)
+      throw std::invalid_argument("one-electron integrals need a one-molecule context (the nuclei would act "
+                                  "on every molecule of the batch)");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const long nc2 = (long)c->ncart * c->ncart;
     c->d_onee.ensure((size_t)3 * nc2 + 4 * (size_t)std::max(1, natom));

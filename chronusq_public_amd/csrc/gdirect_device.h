@@ -34,6 +34,7 @@ struct PairInfo {
   int s1, s2;      // canonical shells (s1 >= s2), as the reference loop visits them
   int p0, p1;      // primitive-pair range in the PrimPair array
   int a0, b0;      // first Cartesian function of a and b
+  int loff;        // first Cartesian function of the pair's molecule (0 for one molecule)
 };
 
 struct QuartetArgs {
@@ -43,11 +44,13 @@ struct QuartetArgs {
   double deg;
   bool ket_uniform;          // every lane of the wavefront holds the same ket pair
   bool bra_swap, ket_swap;   // pair held reversed relative to its canonical (s1 >= s2) order
+  int loff;                  // LDS row origin: first Cartesian function of the molecule
 };
 
 #define MAX_SLOTS 2
 struct JKArgs {
   int ncart;
+  int lds_n;                     // rows of an LDS accumulator column: the largest molecule's ncart
   const double* D[MAX_SLOTS];    // density in the Cartesian basis, row-major ncart x ncart
   double* GJ[MAX_SLOTS];         // forward Coulomb updates
   double* GJ2[MAX_SLOTS];        // forward Coulomb updates from the transposed density

@@ -30,6 +30,7 @@ __device__ __forceinline__ void make_quartet(const uint2* __restrict__ tasks, lo
   q.ket_uniform = (__ballot(tk.y == k0) == ~0ull);
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
+  q.loff = K.loff;
 }
 
 // Quartet of one lane's bra (pair B, task code bra_code) with the wavefront-uniform ket pair
@@ -55,6 +56,7 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
   q.ket_uniform = true;
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
+  q.loff = K.loff;   // the uniform ket's molecule (every quartet of a segment lies in it)
 }
 
 // Quartet of the wavefront-uniform bra pair bb and its primitive block bblk (from
@@ -79,6 +81,7 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
   q.ket_uniform = false;
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
+  q.loff = B.loff;   // the uniform bra's molecule
 }
 
 // JK kernel: work items are (chunk, 64-task block) pairs, one chunk per wavefront, so classes
@@ -136,10 +139,10 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
     const long nitems = (long)(*nseg_ptr) * NCH;                                              \
     if (nitems == 0) return;                                                                  \
     stage_boys(boys, LTOT, boys_lds);                                                         \
-    const int N = jk.ncart;                                                                   \
+    const int N = jk.ncart, LN = jk.lds_n;                                                    \
     double* colK = dyn_lds;                                                                   \
-    double* jcdL = dyn_lds + NCOL * N;                                                        \
-    for (int k = threadIdx.x; k < NCOL * N + NC * ND; k += WGS) dyn_lds[k] = 0.0;             \
+    double* jcdL = dyn_lds + NCOL * LN;                                                       \
+    for (int k = threadIdx.x; k < NCOL * LN + NC * ND; k += WGS) dyn_lds[k] = 0.0;            \
     const int wofs = threadIdx.x & ~63;                                                       \
     while (true) {                                                                            \
       __syncthreads();                                                                        \
@@ -163,10 +166,10 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       }                                                                                       \
       __syncthreads();                                                                        \
       const PairInfo K = pairs[kk];                                                           \
-      for (int k = threadIdx.x; k < NCOL * N; k += WGS) {                                      \
+      for (int k = threadIdx.x; k < NCOL * LN; k += WGS) {                                     \
         const double v = colK[k];                                                             \
         if (v != 0.0) {                                                                       \
-          const int col = k / N, row = k - col * N;                                           \
+          const int col = k / LN, row = k - col * LN + K.loff;                                \
           const int gcol = col < NC ? K.a0 + col : K.b0 + (col - NC);                         \
           atomicAdd(&jk.GK[0][(long)row * N + gcol], v);                                      \
           colK[k] = 0.0;                                                                      \
@@ -221,10 +224,10 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
     const long nitems = (long)(*nseg_ptr) * NCH;                                                \
     if (nitems == 0) return;                                                                    \
     stage_boys(boys, LTOT, boys_lds);                                                           \
-    const int N = jk.ncart;                                                                     \
+    const int N = jk.ncart, LN = jk.lds_n;                                                      \
     double* rowK = dyn_lds;                                                                     \
-    double* jabL = dyn_lds + NROW * N;                                                          \
-    for (int k = threadIdx.x; k < NROW * N + NA * NB; k += WG) dyn_lds[k] = 0.0;               \
+    double* jabL = dyn_lds + NROW * LN;                                                         \
+    for (int k = threadIdx.x; k < NROW * LN + NA * NB; k += WG) dyn_lds[k] = 0.0;              \
     const int wofs = threadIdx.x & ~63;                                                         \
     while (true) {                                                                              \
       __syncthreads();                                                                          \
@@ -248,10 +251,10 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       }                                                                                         \
       __syncthreads();                                                                          \
       const PairInfo A = pairs[bb];                                                             \
-      for (int k = threadIdx.x; k < NROW * N; k += WG) {                                        \
+      for (int k = threadIdx.x; k < NROW * LN; k += WG) {                                       \
         const double v = rowK[k];                                                               \
         if (v != 0.0) {                                                                         \
-          const int lrow = k / N, col = k - lrow * N;                                           \
+          const int lrow = k / LN, col = k - lrow * LN + A.loff;                                \
           const int grow = lrow < NA ? A.a0 + lrow : A.b0 + (lrow - NA);                        \
           atomicAdd(&jk.GK[0][(long)grow * N + col], v);                                        \
           rowK[k] = 0.0;                                                                        \

@@ -57,6 +57,15 @@ int cq_ctx_create(cq_ctx_t** ctx, int nshell, const int* l, const int* pure, con
                   const double* alpha, const double* coef, const double* origin);
 int cq_ctx_destroy(cq_ctx_t* ctx);
 
+/* A context for several molecules at once (no reference counterpart; the reference runs one
+ * molecule per job): the shells of all molecules concatenated, molecule[s] = 0, 1, ...
+ * nondecreasing over the shells.  Shell pairs and quartets are formed only within a molecule,
+ * so one contraction over the block-diagonal N x N density (N = all molecules' functions)
+ * gives every molecule's own G[D] in its diagonal block, with one launch per class pair for
+ * the whole batch.  cq_one_electron rejects such a context. */
+int cq_ctx_create_batch(cq_ctx_t** ctx, int nshell, const int* l, const int* pure, const int* nprim,
+                        const double* alpha, const double* coef, const double* origin, const int* molecule);
+
 /* Message for the last failing call on ctx (or of cq_ctx_create when ctx is NULL). */
 const char* cq_last_error(const cq_ctx_t* ctx);
 

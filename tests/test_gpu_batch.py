@@ -57,3 +57,63 @@ def test_batch_rejects_open_shells():
     oh = M.Molecule.from_angstrom(["O", "H"], [[0, 0, 0], [0, 0, 0.97]])
     with pytest.raises(ValueError):
         BatchSCF([oh], "6-31g*")
+
+
+def test_batch_context_gives_every_molecules_own_gd():
+    """cq_ctx_create_batch: one contraction over the block-diagonal density of three molecules
+    gives, in each diagonal block, that molecule's own J and K (1e-10), zeros off the
+    diagonal blocks, and the molecules' surviving-quartet counts summed; the one-electron
+    integrals refuse a batch context."""
+    import torch
+    from chronusq_public_amd import basis as B
+    from chronusq_public_amd._lib import CQError
+    from chronusq_public_amd.aointegrals import AOIntegrals, COULOMB, EXCHANGE
+    mols = [M.water_test0001(), M.alkane(2), M.water_cluster(2, 3, box_ang=M.scaled_box(2))]
+    bases = [B.build_basis("6-31g*", m.Z, m.coords_bohr) for m in mols]
+    big = AOIntegrals.batch(bases)
+    big.thresholdSchwartz = 1e-12
+    n = big.nBasis
+    assert n == sum(b.nbasis for b in bases)
+    rng = np.random.default_rng(3)
+    X = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    blocks, off = [], 0
+    for b in bases:
+        A = rng.uniform(-1, 1, (b.nbasis, b.nbasis))
+        A = 0.5 * (A + A.T)
+        blocks.append((off, b.nbasis, A))
+        X[off:off + b.nbasis, off:off + b.nbasis] = torch.from_numpy(A)
+        off += b.nbasis
+    J = torch.zeros_like(X)
+    K = torch.zeros_like(X)
+    big.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+    nq = sum(big.quartet_counts().values())
+    mask = torch.zeros_like(X, dtype=torch.bool)
+    total = 0
+    for (o, nb, A), b in zip(blocks, bases):
+        ao = AOIntegrals(b)
+        ao.thresholdSchwartz = 1e-12
+        x = torch.from_numpy(A).cuda()
+        j = torch.zeros_like(x)
+        k = torch.zeros_like(x)
+        ao.newTwoEContractDirect([x, x], [j, k], [COULOMB, EXCHANGE], [1.0, -0.5])
+        total += sum(ao.quartet_counts().values())
+        assert float((J[o:o + nb, o:o + nb] - j).abs().max()) < 1e-10
+        assert float((K[o:o + nb, o:o + nb] - k).abs().max()) < 1e-10
+        mask[o:o + nb, o:o + nb] = True
+    assert float(J[~mask].abs().max()) == 0.0 and float(K[~mask].abs().max()) == 0.0
+    assert nq == total
+    S = torch.zeros_like(X)
+    with pytest.raises(CQError):
+        big.computeAOOneE(mols[0])
+
+
+def test_batch_per_molecule_calls_match_merged():
+    from chronusq_public_amd.batch import BatchSCF
+    mols = _uneven()[:4]
+    a = BatchSCF(mols, "6-31g*", merged=True, group_functions=40)   # several groups
+    ra = a.run()
+    b = BatchSCF(mols, "6-31g*", merged=False)
+    rb = b.run()
+    assert len(a.groups) > 1
+    assert ra.converged.all() and rb.converged.all()
+    assert np.abs(ra.energies - rb.energies).max() < 1e-9
