@@ -26,7 +26,7 @@ EXPORTED = [
     "cq_get_schwarz_threshold", "cq_set_precision_mode", "cq_get_dims", "cq_get_schwarz",
     "cq_contract_direct", "cq_contract_direct_device", "cq_get_quartet_counts",
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
-    "cq_get_kernel_times", "cq_get_work_stats", "cq_one_electron", "cq_set_launch_streams", "cq_set_lds_digestion",
+    "cq_get_kernel_times", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_set_launch_streams", "cq_set_lds_digestion",
     "cq_solid_harmonics", "cq_set_class_digestion", "cq_get_class_digestion",
     "cq_set_class_stream",
     # learned initial guess (include/cqgpu_guess.h)
@@ -79,6 +79,7 @@ def load():
     L.cq_set_launch_streams.argtypes = [vp, ctypes.c_int]
     L.cq_set_lds_digestion.argtypes = [vp, ctypes.c_int]
     L.cq_one_electron.argtypes = [vp, ctypes.c_int, dp, dp, vp, vp, vp, ctypes.c_int, vp]
+    L.cq_one_electron_batch.argtypes = [vp, ctypes.c_int, dp, dp, ip, vp, vp, vp, ctypes.c_int, vp]
     L.cq_get_kernel_times.argtypes = [vp, dp, P(ctypes.c_longlong), ctypes.c_int]
     L.cq_get_work_stats.argtypes = [vp, dp, ctypes.c_int]
     L.cq_solid_harmonics.argtypes = [ctypes.c_int, dp]

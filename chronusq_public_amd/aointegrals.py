@@ -245,6 +245,23 @@ class AOIntegrals:
                                           N, ctypes.c_void_p(stream)), self._h)
         return tuple(out)
 
+    def computeAOOneE_batch(self, mols, device=None):
+        """S, T, V of a batch context (AOIntegrals.batch) for the molecules in the same order:
+        N x N tensors holding every molecule's own integrals in its diagonal block
+        (cq_one_electron_batch).  No reference counterpart."""
+        import torch
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        N = self.nBasis
+        out = [torch.empty((N, N), dtype=torch.float64, device=dev) for _ in range(3)]
+        Z = np.ascontiguousarray(np.concatenate([m.Z for m in mols]), dtype=np.float64)
+        xyz = np.ascontiguousarray(np.concatenate([m.coords_bohr for m in mols]), dtype=np.float64)
+        amol = np.ascontiguousarray(np.concatenate([np.full(m.natom, k, dtype=np.int32) for k, m in enumerate(mols)]))
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        check(_lib.load().cq_one_electron_batch(self._h, len(Z), _dp(Z), _dp(xyz), _ip(amol),
+                                                ctypes.c_void_p(out[0].data_ptr()), ctypes.c_void_p(out[1].data_ptr()),
+                                                ctypes.c_void_p(out[2].data_ptr()), N, ctypes.c_void_p(stream)), self._h)
+        return tuple(out)
+
     # ------------------------------------------------------------------ diagnostics
     def quartet_counts(self) -> dict:
         L = _lib.load()

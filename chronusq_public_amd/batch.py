@@ -94,26 +94,36 @@ class BatchSCF:
         T = torch.zeros((Bn, Nm, Nm), **f64)
         V = torch.zeros((Bn, Nm, Nm), **f64)
         L = _lib.load()
+        self.merged = merged
+        self.groups = []
+        self.ao = []
+        if merged:
+            # one batch context per group of molecules (G[D] and the one-electron integrals)
+            self._make_groups(group_functions, setup_threads)
+            for g in self.groups:
+                Sg, Tg, Vg = g["ao"].computeAOOneE_batch([self.mols[k] for k in g["members"]], self.dev)
+                for dst, src in ((self.S, Sg), (T, Tg), (V, Vg)):
+                    dst.view(-1)[g["pad"]] = src.view(-1)[g["dense"]]
+        else:
+            def make(k):
+                # contexts (pair lists, Schwarz bounds) and one-electron integrals per molecule; the
+                # library calls release the interpreter lock, so threads overlap them
+                torch.cuda.set_device(self.dev)
+                m, bs = self.mols[k], self.bases[k]
+                ao = AOIntegrals(bs)
+                ao.thresholdSchwartz = tau
+                _lib.check(L.cq_set_launch_streams(ao._h, launch_streams), ao._h)
+                Z = np.ascontiguousarray(m.Z, dtype=np.float64)
+                xyz = np.ascontiguousarray(m.coords_bohr, dtype=np.float64)
+                _lib.check(L.cq_one_electron(ao._h, m.natom, Z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             ctypes.c_void_p(self.S[k].data_ptr()), ctypes.c_void_p(T[k].data_ptr()),
+                                             ctypes.c_void_p(V[k].data_ptr()), Nm, None), ao._h)
+                return ao
 
-        def make(k):
-            # contexts (pair lists, Schwarz bounds) and one-electron integrals per molecule; the
-            # library calls release the interpreter lock, so threads overlap them
-            torch.cuda.set_device(self.dev)
-            m, bs = self.mols[k], self.bases[k]
-            ao = AOIntegrals(bs)
-            ao.thresholdSchwartz = tau
-            _lib.check(L.cq_set_launch_streams(ao._h, launch_streams), ao._h)
-            Z = np.ascontiguousarray(m.Z, dtype=np.float64)
-            xyz = np.ascontiguousarray(m.coords_bohr, dtype=np.float64)
-            _lib.check(L.cq_one_electron(ao._h, m.natom, Z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                         xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                         ctypes.c_void_p(self.S[k].data_ptr()), ctypes.c_void_p(T[k].data_ptr()),
-                                         ctypes.c_void_p(V[k].data_ptr()), Nm, None), ao._h)
-            return ao
-
-        torch.cuda.synchronize(self.dev)    # S, T, V zeroed before the contexts' streams write them
-        with ThreadPoolExecutor(max_workers=setup_threads) as ex:
-            self.ao = list(ex.map(make, range(Bn)))
+            torch.cuda.synchronize(self.dev)    # S, T, V zeroed before the contexts' streams write them
+            with ThreadPoolExecutor(max_workers=setup_threads) as ex:
+                self.ao = list(ex.map(make, range(Bn)))
         self.H = T + V
         self.real = torch.zeros((Bn, Nm), dtype=torch.bool, device=self.dev)
         for k, n in enumerate(self.N):
@@ -123,20 +133,14 @@ class BatchSCF:
         self.pad_level = torch.diag_embed(pad * PAD_LEVEL)
         self.occ = (torch.arange(Nm, device=self.dev)[None, :] < torch.from_numpy(self.nocc).to(self.dev)[:, None])
         self.X = self._lowdin(self.S)
-        # merged: one batch context (cq_ctx_create_batch) per group of molecules of at most
-        # group_functions functions; each G[D] of the group is one contraction over the
-        # block-diagonal density, one launch per class pair for all its molecules
-        self.merged = merged
-        self.groups = []
-        if merged:
-            self._make_groups(group_functions)
-            for ao in self.ao:
-                ao.close()            # the per-molecule contexts served the one-electron integrals
-            self.ao = []
         torch.cuda.synchronize()
         self.setup_s = time.perf_counter() - t0
 
-    def _make_groups(self, max_functions: int):
+    def _make_groups(self, max_functions: int, threads: int = 8):
+        """merged: one batch context (cq_ctx_create_batch) per group of molecules of at most
+        max_functions functions, created on `threads` threads; each G[D] of a group is one
+        contraction over its block-diagonal density, one launch per class pair for all its
+        molecules."""
         torch = self.torch
         members, cur, tot = [], [], 0
         for k in range(len(self.mols)):
@@ -148,9 +152,16 @@ class BatchSCF:
         if cur:
             members.append(cur)
         Nm = self.Nmax
-        for mem in members:
+
+        def make(mem):
+            torch.cuda.set_device(self.dev)
             ao = AOIntegrals.batch([self.bases[k] for k in mem])
             ao.thresholdSchwartz = self.tau
+            return ao
+
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            aos = list(ex.map(make, members))
+        for mem, ao in zip(members, aos):
             n = ao.nBasis
             dense, pad = [], []
             off = 0

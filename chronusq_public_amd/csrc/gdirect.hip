@@ -1248,6 +1248,74 @@ int guarded(cq_ctx* c, F&& f) {
 
 thread_local std::string g_create_error;
 
+// S, T, V of the context's shell pairs into ld-strided n x n outputs (column-major, symmetric);
+// with atom_mol (nuclei grouped by molecule, ids nondecreasing) each molecule's pairs feel only
+// its own nuclei.  Synchronous.
+void one_electron(cq_ctx* c, int natom, const double* Z, const double* xyz, const int* atom_mol, double* S,
+                  double* T, double* V, int ld, void* stream) {
+  if (natom < 0 || (natom > 0 && (!Z || !xyz)) || !S || !T || !V || ld < c->nsph)
+    throw std::invalid_argument("invalid one-electron arguments");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  // nuclei per molecule
+  std::vector<int> mol_atom0(c->nmol + 1, 0);
+  int max_atoms = natom;
+  if (atom_mol) {
+    for (int a = 0; a < natom; ++a) {
+      if (atom_mol[a] < 0 || atom_mol[a] >= c->nmol || (a > 0 && atom_mol[a] < atom_mol[a - 1]))
+        throw std::invalid_argument("atom_molecule must be nondecreasing molecule ids of the context");
+      mol_atom0[atom_mol[a] + 1]++;
+    }
+    max_atoms = 0;
+    for (int m = 0; m < c->nmol; ++m) {
+      max_atoms = std::max(max_atoms, mol_atom0[m + 1]);
+      mol_atom0[m + 1] += mol_atom0[m];
+    }
+  }
+  const long nc2 = (long)c->ncart * c->ncart;
+  const int na1 = std::max(1, natom);
+  c->d_onee.ensure((size_t)3 * nc2 + 4 * (size_t)na1);
+  double* Sc = c->d_onee.p;
+  double* Tc = Sc + nc2;
+  double* Vc = Tc + nc2;
+  double* dZ = Vc + nc2;
+  double* dxyz = dZ + na1;
+  std::vector<double> zx((size_t)4 * na1, 0.0);
+  for (int a = 0; a < natom; ++a) {
+    zx[a] = Z[a];
+    for (int k = 0; k < 3; ++k) zx[na1 + 3 * a + k] = xyz[3 * a + k];
+  }
+  HIPCHK(hipMemcpyAsync(dZ, zx.data(), zx.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  OneEShellData sd{c->d_prim_off.p, c->d_alpha.p, c->d_coef.p, c->d_origin.p};
+  DevBuf<int> d_smol, d_mat0;
+  if (atom_mol) {
+    d_smol.upload(c->shell_mol, s);
+    d_mat0.upload(mol_atom0, s);
+    sd.shell_mol = d_smol.p;
+    sd.mol_atom0 = d_mat0.p;
+  }
+  ZeroList zl{};
+  zl.p[0] = (unsigned*)Sc;
+  zl.n[0] = 3 * nc2 * 2;
+  zl.count = 1;
+  hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(WG), 0, s, zl);
+  int max_prim = 1;
+  for (size_t k = 0; k + 1 < c->prim_off_host.size(); ++k)
+    max_prim = std::max(max_prim, c->prim_off_host[k + 1] - c->prim_off_host[k]);
+  for (int X = 0; X < c->npc; ++X)
+    launch_onee(X, c->d_pairs_1e.p, c->class_start_1e[X], c->class_start_1e[X + 1], sd, max_prim, dZ, dxyz, natom,
+                max_atoms, c->d_boys.p, c->ncart, Sc, Tc, Vc, s);
+  HIPCHK(hipGetLastError());
+  double* outs[3] = {S, T, V};
+  for (int k = 0; k < 3; ++k)
+    hipLaunchKernelGGL(sph_transform_kernel, dim3(grid_for((long)c->nsph * c->nsph, WG)), dim3(WG), 0, s,
+                       c->d_onee.p + k * nc2, outs[k], c->nsph, ld, c->d_shells.p, c->d_sph2shell.p, c->ncart,
+                       c->d_tco.p, c->d_tco_off.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));   // the host staging vectors and device tables are released on return
+  d_smol.release();
+  d_mat0.release();
+}
+
 }  // namespace
 
 // =======================================================================================
@@ -1412,45 +1480,18 @@ int cq_solid_harmonics(int l, double* out) {
 int cq_one_electron(cq_ctx_t* c, int natom, const double* Z, const double* xyz, double* S, double* T, double* V,
                     int ld, void* stream) {
   return guarded(c, [&] {
-    if (natom < 0 || (natom > 0 && (!Z || !xyz)) || !S || !T || !V || ld < c->nsph)
-      throw std::invalid_argument("invalid one-electron arguments");
     if (c->nmol > 1)
-      throw std::invalid_argument("one-electron integrals need a one-molecule context (the nuclei would act "
-                                  "on every molecule of the batch)");
-    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    const long nc2 = (long)c->ncart * c->ncart;
-    c->d_onee.ensure((size_t)3 * nc2 + 4 * (size_t)std::max(1, natom));
-    double* Sc = c->d_onee.p;
-    double* Tc = Sc + nc2;
-    double* Vc = Tc + nc2;
-    double* dZ = Vc + nc2;
-    double* dxyz = dZ + std::max(1, natom);
-    std::vector<double> zx((size_t)4 * std::max(1, natom), 0.0);
-    for (int a = 0; a < natom; ++a) {
-      zx[a] = Z[a];
-      for (int k = 0; k < 3; ++k) zx[std::max(1, natom) + 3 * a + k] = xyz[3 * a + k];
-    }
-    HIPCHK(hipMemcpyAsync(dZ, zx.data(), zx.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    OneEShellData sd{c->d_prim_off.p, c->d_alpha.p, c->d_coef.p, c->d_origin.p};
-    ZeroList zl{};
-    zl.p[0] = (unsigned*)Sc;
-    zl.n[0] = 3 * nc2 * 2;
-    zl.count = 1;
-    hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(WG), 0, s, zl);
-    int max_prim = 1;
-    for (size_t k = 0; k + 1 < c->prim_off_host.size(); ++k)
-      max_prim = std::max(max_prim, c->prim_off_host[k + 1] - c->prim_off_host[k]);
-    for (int X = 0; X < c->npc; ++X)
-      launch_onee(X, c->d_pairs_1e.p, c->class_start_1e[X], c->class_start_1e[X + 1], sd, max_prim, dZ, dxyz, natom,
-                  c->d_boys.p, c->ncart, Sc, Tc, Vc, s);
-    HIPCHK(hipGetLastError());
-    double* outs[3] = {S, T, V};
-    for (int k = 0; k < 3; ++k)
-      hipLaunchKernelGGL(sph_transform_kernel, dim3(grid_for((long)c->nsph * c->nsph, WG)), dim3(WG), 0, s,
-                         c->d_onee.p + k * nc2, outs[k], c->nsph, ld, c->d_shells.p, c->d_sph2shell.p, c->ncart,
-                         c->d_tco.p, c->d_tco_off.p);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s));   // the host staging vector is released on return
+      throw std::invalid_argument("a context of several molecules needs cq_one_electron_batch (which nuclei "
+                                  "belong to which molecule)");
+    one_electron(c, natom, Z, xyz, nullptr, S, T, V, ld, stream);
+  });
+}
+
+int cq_one_electron_batch(cq_ctx_t* c, int natom, const double* Z, const double* xyz, const int* atom_molecule,
+                          double* S, double* T, double* V, int ld, void* stream) {
+  return guarded(c, [&] {
+    if (!atom_molecule) throw std::invalid_argument("atom_molecule is required");
+    one_electron(c, natom, Z, xyz, atom_molecule, S, T, V, ld, stream);
   });
 }
 

@@ -10,11 +10,16 @@ struct OneEShellData {
   const double* alpha;
   const double* coef;
   const double* origin;
+  // several molecules: shell -> molecule, and molecule m's nuclei [mol_atom0[m], mol_atom0[m+1])
+  // (both nullptr for one molecule: every nucleus acts on every pair)
+  const int* shell_mol = nullptr;
+  const int* mol_atom0 = nullptr;
 };
 
 // S, T, V Cartesian blocks of the shell pairs [p_begin, p_end) of one pair class
 // (0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd), added into zeroed row-major ncart x ncart matrices;
-// max_prim bounds the primitive count of any shell.
+// max_prim bounds the primitive count of any shell, max_atoms the nuclei acting on one pair
+// (natom, or the largest molecule's atom count).
 void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
-                 int max_prim, const double* Z, const double* xyz, int natom, const double* boys, int ncart,
-                 double* S, double* T, double* V, hipStream_t stream);
+                 int max_prim, const double* Z, const double* xyz, int natom, int max_atoms, const double* boys,
+                 int ncart, double* S, double* T, double* V, hipStream_t stream);

@@ -88,8 +88,16 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
   const int ip0 = DO_V ? sd.prim_off[a] : sd.prim_off[a] + (int)blockIdx.y;
   const int ip1 = DO_V ? sd.prim_off[a + 1] : ip0 + 1;
   if (ip0 >= sd.prim_off[a + 1]) return;
-  const int c0 = DO_V ? (int)blockIdx.y * ONEE_ATOMS : 0;
-  const int c1 = DO_V ? min(natom, c0 + ONEE_ATOMS) : 0;
+  // nuclei of this slice: those of the pair's molecule when several molecules share the context
+  int at0 = 0, at1 = natom;
+  if (DO_V && sd.mol_atom0) {
+    const int m = sd.shell_mol[a];
+    at0 = sd.mol_atom0[m];
+    at1 = sd.mol_atom0[m + 1];
+  }
+  const int c0 = DO_V ? at0 + (int)blockIdx.y * ONEE_ATOMS : 0;
+  const int c1 = DO_V ? min(at1, c0 + ONEE_ATOMS) : 0;
+  if (DO_V && c0 >= at1) return;
   const double A[3] = {sd.origin[3 * a], sd.origin[3 * a + 1], sd.origin[3 * a + 2]};
   const double B[3] = {sd.origin[3 * b], sd.origin[3 * b + 1], sd.origin[3 * b + 2]};
   const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
@@ -207,12 +215,12 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
 }  // namespace
 
 void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
-                 int max_prim, const double* Z, const double* xyz, int natom, const double* boys, int ncart,
-                 double* S, double* T, double* V, hipStream_t stream) {
+                 int max_prim, const double* Z, const double* xyz, int natom, int max_atoms, const double* boys,
+                 int ncart, double* S, double* T, double* V, hipStream_t stream) {
   const int n = p_end - p_begin;
   if (n <= 0) return;
   const dim3 grid_st((n + WG - 1) / WG, std::max(1, max_prim));
-  const dim3 grid_v((n + WG - 1) / WG, std::max(1, (natom + ONEE_ATOMS - 1) / ONEE_ATOMS));
+  const dim3 grid_v((n + WG - 1) / WG, std::max(1, (max_atoms + ONEE_ATOMS - 1) / ONEE_ATOMS));
   const dim3 block(WG);
 #define CQ_ONEE(LA, LB)                                                                                 \
   do {                                                                                                   \

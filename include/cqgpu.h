@@ -106,6 +106,13 @@ int cq_contract_direct_device(cq_ctx_t* ctx, int nmat, const double* const* X, d
 int cq_one_electron(cq_ctx_t* ctx, int natom, const double* Z, const double* xyz, double* S, double* T,
                     double* V, int ld, void* stream);
 
+/* The same for a context of several molecules (cq_ctx_create_batch): atom_molecule[natom] gives
+ * each nucleus's molecule (nondecreasing), and every molecule's shell pairs feel only its own
+ * nuclei, so the outputs are the molecules' own S, T, V in their diagonal blocks.  No
+ * reference counterpart. */
+int cq_one_electron_batch(cq_ctx_t* ctx, int natom, const double* Z, const double* xyz, const int* atom_molecule,
+                          double* S, double* T, double* V, int ld, void* stream);
+
 /* Real solid-harmonic coefficients for angular momentum l (0..2): (2l+1) x ncart(l) row-major,
  * m = -l..l, in the shell's uniformly normalised Cartesian functions (x^l ... z^l) -- the
  * transform of every pure shell (cart2sphCoeff, src/basisset/basisset_read.cpp:871-990). */

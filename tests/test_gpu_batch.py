@@ -102,9 +102,15 @@ def test_batch_context_gives_every_molecules_own_gd():
         mask[o:o + nb, o:o + nb] = True
     assert float(J[~mask].abs().max()) == 0.0 and float(K[~mask].abs().max()) == 0.0
     assert nq == total
-    S = torch.zeros_like(X)
     with pytest.raises(CQError):
         big.computeAOOneE(mols[0])
+    # the batch's one-electron integrals: each molecule's own S, T, V in its diagonal block
+    Sb, Tb, Vb = big.computeAOOneE_batch(mols)
+    for (o, nb, _), b, m in zip(blocks, bases, mols):
+        for got, ref in zip((Sb, Tb, Vb), AOIntegrals(b).computeAOOneE(m)):
+            assert float((got[o:o + nb, o:o + nb] - ref).abs().max()) < 1e-12
+    for M_ in (Sb, Tb, Vb):
+        assert float(M_[~mask].abs().max()) == 0.0
 
 
 def test_batch_per_molecule_calls_match_merged():
