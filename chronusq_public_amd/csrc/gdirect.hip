@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -45,7 +46,37 @@
 static ClassKernels kClassKernels[CQ_NUM_CLASS_PAIRS];
 // dynamic LDS of the segment kernels: what the 64 KiB per workgroup leaves beside the static
 // LDS (Boys table, 25992 B, and the work-item slot)
-static const size_t kMaxDynLds = 64 * 1024 - BOYS_NGRID * BOYS_LDS_W * sizeof(double) - 256;
+static const size_t kStaticLds = BOYS_NGRID * BOYS_LDS_W * sizeof(double) + 256;
+static const size_t kMaxDynLds = 64 * 1024 - kStaticLds;
+
+// Dynamic LDS the segment kernels may use on this device: the device's per-workgroup limit
+// (160 KiB on gfx950) less their static LDS, once every segment kernel has been allowed that
+// much (hipFuncSetAttribute); 64 KiB less the static LDS if that fails.  Computed once.
+static size_t device_dyn_lds() {
+  static size_t v = 0;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    v = kMaxDynLds;
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return;
+    if ((size_t)lim <= 64 * 1024) return;
+    const size_t want = (size_t)lim - kStaticLds;
+    for (int k = 0; k < CQ_NUM_CLASS_PAIRS; ++k) {
+      const ClassKernels& ck = kClassKernels[k];
+      for (SegKernel f : {ck.jkseg, ck.jksegb, ck.jkseg32, ck.jksegb32}) {
+        if (!f) continue;
+        if (hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want) != hipSuccess) {
+          (void)hipGetLastError();
+          return;
+        }
+      }
+    }
+    v = want;
+  });
+  return v;
+}
 static const int kNumClassPairs = CQ_NUM_CLASS_PAIRS;
 static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 
@@ -580,6 +611,7 @@ struct cq_ctx {
   DevBuf<int> d_grp, d_cls_grp_off;
   // several molecules in one context: shell pairs and quartets only within a molecule
   int nmol = 1, lds_n = 0;            // lds_n: the largest molecule's Cartesian functions
+  size_t max_dyn_lds = 0;             // dynamic LDS bytes a segment kernel may use
   std::vector<int> shell_mol;
   std::vector<int> cmg;               // npc x (nmol + 1): first screening group of each molecule
   DevBuf<int> d_pair_mol, d_cmg;
@@ -662,6 +694,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   }
   c->nsph = sph;
   c->ncart = cart;
+  c->max_dyn_lds = device_dyn_lds();
   // molecules: consecutive shells, ids 0, 1, ... in order (nullptr: one molecule)
   c->shell_mol.assign(nshell, 0);
   if (mol) {
@@ -974,7 +1007,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
       for (int q = 0; q < nm; ++q) cap_ket += bsum_m[q] * nket_m[q];
     }
     const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
-    const int flip = (!m.same && ck.jksegb && ldsb <= kMaxDynLds && 2LL * m.nbra <= m.nket) ? 1 : 0;
+    const int flip = (!m.same && ck.jksegb && ldsb <= c->max_dyn_lds && 2LL * m.nbra <= m.nket) ? 1 : 0;
     long long cap_bra = 0;
     if (flip)
       for (int q = 0; q < nm; ++q) cap_bra += ksum_m[q] * nbra_m[q];
@@ -1170,7 +1203,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
       // one density slot and LDS room for the K columns: segment kernel (LDS accumulation)
       const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->lds_n + (size_t)ck.nc * ck.nd) * sizeof(double);
-      const bool seg_path = var >= 1 && ng == 1 && lds_bytes <= kMaxDynLds;
+      const bool seg_path = var >= 1 && ng == 1 && lds_bytes <= c->max_dyn_lds;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (c->profiling) {
         e0 = c->get_event();

@@ -273,3 +273,27 @@ def test_precision_mode_errors():
         ao.setPrecision("half")
     with pytest.raises(CQError):
         ao.setPrecision("mixed", 0.0)
+
+
+def test_large_basis_lds_segments_match_hbm_digestion():
+    """(H2O)32 / cc-pVDZ (768 functions, 800 Cartesian): LDS accumulators of up to 12 columns x
+    800 rows (beyond 64 KiB for the d classes, within the device's per-workgroup limit) give the
+    same J and K as the HBM-atomic digestion."""
+    import torch
+    from chronusq_public_amd import _lib
+    mol = M.water_cluster(32, seed=0, box_ang=M.scaled_box(32))
+    bs = B.build_basis("cc-pvdz", mol.Z, mol.coords_bohr)
+    X = torch.from_numpy(np.ascontiguousarray(_sym_random(bs.nbasis, 9))).cuda()
+    out = []
+    for mode in (2, 0):
+        ao = AOIntegrals(bs)
+        ao.thresholdSchwartz = 1e-12
+        _lib.check(_lib.load().cq_set_lds_digestion(ao._h, mode), ao._h)
+        J = torch.zeros_like(X)
+        K = torch.zeros_like(X)
+        ao.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+        out.append((J, K, sum(ao.quartet_counts().values())))
+    (J2, K2, n2), (J0, K0, n0) = out
+    assert n2 == n0
+    assert float((J2 - J0).abs().max()) < 1e-9 * max(1.0, float(J0.abs().max()))
+    assert float((K2 - K0).abs().max()) < 1e-9 * max(1.0, float(K0.abs().max()))
