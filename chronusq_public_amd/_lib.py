@@ -28,7 +28,7 @@ EXPORTED = [
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
     "cq_get_kernel_times", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_set_launch_streams", "cq_set_lds_digestion",
     "cq_solid_harmonics", "cq_set_class_digestion", "cq_get_class_digestion",
-    "cq_set_class_stream",
+    "cq_set_class_stream", "cq_set_task_budget", "cq_get_task_plan",
     # learned initial guess (include/cqgpu_guess.h)
     "cq_mlp_last_error", "cq_mlp_gemm_bf16", "cq_mlp_relu_backward", "cq_mlp_column_sum", "cq_mlp_to_bf16",
     "cq_mlp_mse", "cq_mlp_adam",
@@ -86,6 +86,8 @@ def load():
     L.cq_set_class_digestion.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.cq_get_class_digestion.argtypes = [vp, ctypes.c_int, ip]
     L.cq_set_class_stream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.cq_set_task_budget.argtypes = [vp, ctypes.c_longlong]
+    L.cq_get_task_plan.argtypes = [vp, P(ctypes.c_longlong), ip]
     ci, cl, cf = ctypes.c_int, ctypes.c_long, ctypes.c_float
     L.cq_mlp_last_error.argtypes = []
     L.cq_mlp_last_error.restype = ctypes.c_char_p

@@ -20,7 +20,7 @@ the GPU; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
-from typing import List, Sequence
+from typing import List, Sequence, Tuple
 
 import numpy as np
 
@@ -171,6 +171,17 @@ class AOIntegrals:
             check(L.cq_get_class_digestion(self._h, k, ctypes.byref(v)), self._h)
             out.append(v.value)
         return out
+
+    def set_task_budget(self, entries: int) -> None:
+        """Largest screened task list of one class pair, in 8-byte entries; class pairs with a
+        larger worst case run in batches of uniform pairs (cq_set_task_budget)."""
+        check(_lib.load().cq_set_task_budget(self._h, int(entries)), self._h)
+
+    def task_plan(self) -> Tuple[int, int]:
+        """(task entries allocated, largest number of batches of one class pair)."""
+        e, b = ctypes.c_longlong(), ctypes.c_int()
+        check(_lib.load().cq_get_task_plan(self._h, ctypes.byref(e), ctypes.byref(b)), self._h)
+        return e.value, b.value
 
     def computeSchwartz(self) -> np.ndarray:
         Q = np.zeros((self.nShell, self.nShell))

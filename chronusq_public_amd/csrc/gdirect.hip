@@ -112,18 +112,19 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     const unsigned char* __restrict__ pair_nblk, const unsigned char* __restrict__ pair_nublk,
     uint2* __restrict__ tasks,
     unsigned long long* __restrict__ counts, unsigned long long* __restrict__ qcounts,
-    Segment* __restrict__ segs, unsigned* __restrict__ nseg, int cp_fixed) {
+    Segment* __restrict__ segs, unsigned* __restrict__ nseg, int cp_fixed, int item0, int count_quartets) {
   __shared__ uint2 buf[WG / 64][SCREEN_BUF];
   const int wid = threadIdx.x >> 6;
   const int wave = blockIdx.x * (WG / 64) + wid;
   const int lane = threadIdx.x & 63;
   if (wave >= nitem) return;
-  // cp_fixed >= 0: this launch screens that class pair only (nitem = its uniform pairs)
+  // cp_fixed >= 0: this launch screens that class pair only, its uniform pairs
+  // item0 .. item0 + nitem - 1 (one batch)
   int cp = cp_fixed >= 0 ? cp_fixed : 0;
   if (cp_fixed < 0)
     while (cp + 1 < ncp && item_start[cp + 1] <= wave) ++cp;
   const CPMeta m = cps[cp];
-  const int ket_local = cp_fixed >= 0 ? wave : wave - item_start[cp];
+  const int ket_local = cp_fixed >= 0 ? item0 + wave : wave - item_start[cp];
   const int K = (m.flip ? m.bra_start : m.ket_start) + ket_local;   // the uniform pair
   const int sw_start = m.flip ? m.ket_start : m.bra_start;           // the swept class
   const int nsw = m.flip ? m.nket : m.nbra;
@@ -200,7 +201,8 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   }
   }
   if (nb > 0) flush();
-  if (lane == 0 && nq > 0) atomicAdd(&qcounts[cp], nq);
+  // (a list screened again for a later slot group is not counted twice)
+  if (lane == 0 && nq > 0 && count_quartets) atomicAdd(&qcounts[cp], nq);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -577,6 +579,13 @@ struct cq_ctx {
   std::vector<int> item_start[2];
   int nitem[2] = {0, 0};
   long long task_capacity = 0;
+  // task lists per class pair are at most task_budget entries (8 B each); a class pair whose
+  // worst case is larger screens and digests its uniform pairs in batches:
+  // batch_start[t][k] holds the class-local uniform-pair boundaries (nbatch + 1 of them)
+  long long task_budget = 1LL << 28;
+  std::vector<std::vector<int>> batch_start[2];
+  std::vector<unsigned char> nblk_h, nublk_h;
+  std::vector<int> pair_mol_h;
   std::vector<unsigned long long> last_counts;
   // profiling: events around every class-pair kernel of every contraction
   bool profiling = false;
@@ -665,6 +674,117 @@ int persistent_grid() {
     g = std::max(1, ncu) * 8;
   }
   return g;
+}
+
+// Class-pair metadata, task-list capacities and batches of uniform pairs.  Table 1 keys the
+// segments of a class pair on the bra pair when the bra class has at most half as many pairs
+// as the ket class (few bras survive per ket, many kets per bra) and the bra-keyed LDS
+// accumulators fit.  A uniform pair's worst case is every swept pair of its own molecule
+// (same class: from the uniform pair on), one task entry per primitive block of a swept pair.
+// Consecutive uniform pairs form one batch while their worst cases sum to at most
+// task_budget, so a class pair's task list never exceeds max(task_budget, one uniform pair's
+// worst case) whatever the density; a batch is screened and digested before the next one.
+void plan_tasks(cq_ctx* c) {
+  const int nm = c->nmol;
+  const std::vector<unsigned char>& nblk = c->nblk_h;
+  const std::vector<unsigned char>& nublk = c->nublk_h;
+  const std::vector<int>& pair_mol = c->pair_mol_h;
+  for (int t = 0; t < 2; ++t) {
+    c->cps[t].clear();
+    c->item_start[t].clear();
+    c->batch_start[t].assign(kNumClassPairs, std::vector<int>());
+  }
+  auto class_max = [&](int cls, const std::vector<unsigned char>& nb) {
+    int mx = 1;
+    for (int k = c->class_start[cls]; k < c->class_start[cls + 1]; ++k) mx = std::max<int>(mx, nb[k]);
+    return mx;
+  };
+  const long long budget = c->task_budget;
+  long long off = 0, seg_total[2] = {0, 0};
+  int items[2] = {0, 0};
+  for (int k = 0; k < kNumClassPairs; ++k) {
+    const ClassKernels& ck = kClassKernels[k];
+    CPMeta m{};
+    m.bra_start = c->class_start[ck.bi];
+    m.nbra = c->class_start[ck.bi + 1] - c->class_start[ck.bi];
+    m.ket_start = c->class_start[ck.ki];
+    m.nket = c->class_start[ck.ki + 1] - c->class_start[ck.ki];
+    m.same = ck.i == ck.j;
+    m.bra_class = ck.bi;
+    m.ket_class = ck.ki;
+    // swept-side primitive blocks per molecule
+    std::vector<long long> bsum_m(nm, 0), ksum_m(nm, 0);
+    for (int i = 0; i < m.nbra; ++i) bsum_m[pair_mol[m.bra_start + i]] += nblk[m.bra_start + i];
+    for (int i = 0; i < m.nket; ++i) ksum_m[pair_mol[m.ket_start + i]] += nblk[m.ket_start + i];
+    const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
+    const int flip = (!m.same && ck.jksegb && ldsb <= c->max_dyn_lds && 2LL * m.nbra <= m.nket) ? 1 : 0;
+    CPMeta mt[2] = {m, m};
+    long long region = 0;
+    for (int t = 0; t < 2; ++t) {
+      mt[t].flip = t == 1 ? flip : 0;
+      const int uni_start = mt[t].flip ? m.bra_start : m.ket_start;
+      const int nuni = mt[t].flip ? m.nbra : m.nket;
+      std::vector<long long> w(nuni);
+      if (m.same) {
+        // swept pairs i >= u of u's molecule (a molecule's pairs are contiguous in a class)
+        long long run = 0;
+        for (int u = nuni - 1; u >= 0; --u) {
+          if (u + 1 < nuni && pair_mol[uni_start + u] != pair_mol[uni_start + u + 1]) run = 0;
+          run += nblk[uni_start + u];
+          w[u] = run;
+        }
+      } else {
+        const std::vector<long long>& sw = mt[t].flip ? ksum_m : bsum_m;
+        for (int u = 0; u < nuni; ++u) w[u] = sw[pair_mol[uni_start + u]];
+      }
+      // segments: a uniform pair with T tasks is flushed at most T / (SCREEN_BUF - 64) + 1 times
+      // (a flush happens once fewer than 64 slots are left), each flush writing one segment per
+      // primitive block of the uniform pair
+      const int umax = class_max(mt[t].flip ? m.bra_class : m.ket_class, nublk);
+      std::vector<int>& bs = c->batch_start[t][k];
+      bs.assign(1, 0);
+      long long cap = 0, cur = 0, seg_cap = 0, seg_cur = 0;
+      auto close = [&](int u) {
+        bs.push_back(u);
+        cap = std::max(cap, cur);
+        seg_cap = std::max(seg_cap, seg_cur + (cur / (SCREEN_BUF - 64) + 1) * umax);
+        cur = 0;
+        seg_cur = 0;
+      };
+      for (int u = 0; u < nuni; ++u) {
+        if (u > bs.back() && cur + w[u] > budget) close(u);
+        cur += w[u];
+        seg_cur += nublk[uni_start + u];
+      }
+      close(nuni);
+      // task offsets within a list are 32-bit (Segment::off)
+      if (cap > 0x7fffffffLL) throw std::invalid_argument("one shell pair's task list exceeds 2^31 entries");
+      mt[t].task_cap = cap;
+      mt[t].seg_cap = seg_cap;
+      mt[t].seg_off = seg_total[t];
+      seg_total[t] += seg_cap;
+      region = std::max(region, cap);
+      c->item_start[t].push_back(items[t]);
+      items[t] += nuni;
+    }
+    for (int t = 0; t < 2; ++t) {
+      mt[t].task_off = off;
+      c->cps[t].push_back(mt[t]);
+    }
+    off += region;
+  }
+  c->task_capacity = off;
+  // one call may take table 0 for some class pairs and table 1 for others (per-class digestion
+  // variants): table 1's segments live after table 0's
+  for (auto& mt : c->cps[1]) mt.seg_off += seg_total[0];
+  for (int t = 0; t < 2; ++t) {
+    c->nitem[t] = items[t];
+    c->d_cps[t].upload(c->cps[t], c->stream);
+    c->d_item_start[t].upload(c->item_start[t], c->stream);
+  }
+  c->d_tasks.ensure((size_t)std::max<long long>(1, off));
+  c->d_segs.ensure((size_t)std::max<long long>(1, seg_total[0] + seg_total[1]));
+  HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
@@ -929,6 +1049,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_cls_grp_off.upload(c->cls_grp_off, c->stream);
   c->d_cmg.upload(c->cmg, c->stream);
   c->d_pair_mol.upload(pair_mol, c->stream);
+  c->pair_mol_h = pair_mol;
   {
     std::vector<int2> s12(npair);
     for (int k = 0; k < npair; ++k) s12[k] = make_int2(c->pairs[k].s1, c->pairs[k].s2);
@@ -941,13 +1062,6 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     c->Q[(size_t)pi_.s1 * nshell + pi_.s2] = c->pairQ[k];
     c->Q[(size_t)pi_.s2 * nshell + pi_.s1] = c->pairQ[k];
   }
-  // class-pair metadata and task capacity; table 1 keys the segments of a class pair on the
-  // bra pair when the bra class has at most half as many pairs as the ket class (few bras
-  // survive per ket, many kets per bra) and the bra-keyed LDS accumulators fit
-  for (int t = 0; t < 2; ++t) {
-    c->cps[t].clear();
-    c->item_start[t].clear();
-  }
   // primitive blocks per pair (1: unsplit) as the per-lane side and as the uniform side
   std::vector<unsigned char> nblk(npair), nublk(npair);
   for (int k = 0; k < npair; ++k) {
@@ -959,87 +1073,9 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   if ((unsigned)npair > PAIR_MASK) throw std::invalid_argument("too many shell pairs");
   c->d_pair_nblk.upload(nblk, c->stream);
   c->d_pair_nublk.upload(nublk, c->stream);
-  auto class_blocks = [&](int cls, const std::vector<unsigned char>& nb, long long& sum, int& mx) {
-    sum = 0;
-    mx = 1;
-    for (int k = c->class_start[cls]; k < c->class_start[cls + 1]; ++k) {
-      sum += nb[k];
-      mx = std::max<int>(mx, nb[k]);
-    }
-  };
-  long long off = 0, seg_total[2] = {0, 0};
-  int items[2] = {0, 0};
-  for (int k = 0; k < kNumClassPairs; ++k) {
-    const ClassKernels& ck = kClassKernels[k];
-    CPMeta m{};
-    m.bra_start = c->class_start[ck.bi];
-    m.nbra = c->class_start[ck.bi + 1] - c->class_start[ck.bi];
-    m.ket_start = c->class_start[ck.ki];
-    m.nket = c->class_start[ck.ki + 1] - c->class_start[ck.ki];
-    m.same = ck.i == ck.j;
-    m.bra_class = ck.bi;
-    m.ket_class = ck.ki;
-    long long bsum, ksum, busum, kusum;
-    int bmax, kmax, bumax, kumax;
-    class_blocks(ck.bi, nblk, bsum, bmax);
-    class_blocks(ck.ki, nblk, ksum, kmax);
-    class_blocks(ck.bi, nublk, busum, bumax);
-    class_blocks(ck.ki, nublk, kusum, kumax);
-    // task entries: every swept pair's blocks against every uniform pair of its molecule
-    // (same class: bra >= ket)
-    std::vector<long long> bsum_m(nm, 0), ksum_m(nm, 0), nbra_m(nm, 0), nket_m(nm, 0);
-    for (int i = 0; i < m.nbra; ++i) {
-      bsum_m[pair_mol[m.bra_start + i]] += nblk[m.bra_start + i];
-      nbra_m[pair_mol[m.bra_start + i]] += 1;
-    }
-    for (int i = 0; i < m.nket; ++i) {
-      ksum_m[pair_mol[m.ket_start + i]] += nblk[m.ket_start + i];
-      nket_m[pair_mol[m.ket_start + i]] += 1;
-    }
-    long long cap_ket = 0;
-    if (m.same) {
-      int first = 0;   // class-local index of the first pair of the current molecule
-      for (int i = 0; i < m.nbra; ++i) {
-        if (i > 0 && pair_mol[m.bra_start + i] != pair_mol[m.bra_start + i - 1]) first = i;
-        cap_ket += (long long)nblk[m.bra_start + i] * (i - first + 1);
-      }
-    } else {
-      for (int q = 0; q < nm; ++q) cap_ket += bsum_m[q] * nket_m[q];
-    }
-    const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
-    const int flip = (!m.same && ck.jksegb && ldsb <= c->max_dyn_lds && 2LL * m.nbra <= m.nket) ? 1 : 0;
-    long long cap_bra = 0;
-    if (flip)
-      for (int q = 0; q < nm; ++q) cap_bra += ksum_m[q] * nbra_m[q];
-    m.task_off = off;
-    off += std::max(cap_ket, cap_bra);
-    for (int t = 0; t < 2; ++t) {
-      CPMeta mt = m;
-      mt.flip = t == 1 ? flip : 0;
-      const int nuni = mt.flip ? m.nbra : m.nket;
-      mt.task_cap = mt.flip ? cap_bra : cap_ket;
-      // segments: a uniform pair with T tasks is flushed at most T / (SCREEN_BUF - 64) + 1 times
-      // (a flush happens once fewer than 64 slots are left), each flush writing one segment per
-      // primitive block of the uniform pair
-      mt.seg_cap = (mt.flip ? busum : kusum) + (mt.task_cap / (SCREEN_BUF - 64) + 1) * (mt.flip ? bumax : kumax);
-      mt.seg_off = seg_total[t];
-      seg_total[t] += mt.seg_cap;
-      c->cps[t].push_back(mt);
-      c->item_start[t].push_back(items[t]);
-      items[t] += nuni;
-    }
-  }
-  c->task_capacity = off;
-  // one call may take table 0 for some class pairs and table 1 for others (per-class digestion
-  // variants): table 1's segments live after table 0's
-  for (auto& mt : c->cps[1]) mt.seg_off += seg_total[0];
-  for (int t = 0; t < 2; ++t) {
-    c->nitem[t] = items[t];
-    c->d_cps[t].upload(c->cps[t], c->stream);
-    c->d_item_start[t].upload(c->item_start[t], c->stream);
-  }
-  c->d_tasks.ensure((size_t)std::max<long long>(1, off));
-  c->d_segs.ensure((size_t)std::max<long long>(1, seg_total[0] + seg_total[1]));
+  c->nblk_h.swap(nblk);
+  c->nublk_h.swap(nublk);
+  plan_tasks(c);
   c->d_nseg.ensure(kNumClassPairs);
   c->d_work_q.ensure(kNumClassPairs);
   c->d_counts.ensure(kNumClassPairs);
@@ -1187,15 +1223,6 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       if (m.nbra == 0 || m.nket == 0) continue;
       hipStream_t sk = c->class_stream[k] >= 0 ? c->aux[c->class_stream[k] % c->n_streams]
                                                : c->aux[launched++ % c->n_streams];
-      if (s0 == 0) {
-        const int nuni = m.flip ? m.nbra : m.nket;
-        hipLaunchKernelGGL(screen_kernel, dim3(grid_for(nuni, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p,
-                           kNumClassPairs, c->d_item_start[tb].p, nuni, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
-                           c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p, c->nmol, c->d_Nmax.p, c->d_nglob.p, ns,
-                           lo, hi, c->d_pair_nblk.p,
-                           c->d_pair_nublk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p, c->d_segs.p, c->d_nseg.p, k);
-        HIPCHK(hipGetLastError());
-      }
       const uint2* tasks = c->d_tasks.p + m.task_off;
       const long long cap = m.task_cap;
       const long long items = ((cap + 63) / 64) * 64 * ck.nchunk;
@@ -1204,33 +1231,63 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       // one density slot and LDS room for the K columns: segment kernel (LDS accumulation)
       const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->lds_n + (size_t)ck.nc * ck.nd) * sizeof(double);
       const bool seg_path = var >= 1 && ng == 1 && lds_bytes <= c->max_dyn_lds;
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (c->profiling) {
-        e0 = c->get_event();
-        e1 = c->get_event();
-        HIPCHK(hipEventRecord(e0, sk));
-      }
-      if (m.flip) {
-        // bra-keyed segments (table 1 only flips where the LDS rows fit)
-        const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
-        const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
-        hipLaunchKernelGGL(fp32 ? ck.jksegb32 : ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
-                           c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
-                           c->d_prims.p, c->d_boys.p, jk);
-      } else if (seg_path) {
-        const int sgrid = (int)std::min<long long>((long long)persistent_grid() * WG / ck.seg_wg, m.seg_cap * ck.nchunk);
-        hipLaunchKernelGGL(fp32 ? ck.jkseg32 : ck.jkseg, dim3(std::max(1, sgrid)), dim3(ck.seg_wg), lds_bytes, sk,
-                           tasks, c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
-                           c->d_prims.p, c->d_boys.p, jk);
-      } else {
-        // (the FP32 round's lists run through the FP64 kernels where the segment path does not fit)
-        hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
-                           c->d_prims.p, c->d_boys.p, jk);
-      }
-      HIPCHK(hipGetLastError());
-      if (c->profiling) {
-        HIPCHK(hipEventRecord(e1, sk));
-        c->prof_events[k].push_back({e0, e1});
+      // batches of uniform pairs, in order on this class pair's stream: a class pair in one
+      // batch screens once and keeps its lists for every slot group; a batched one clears its
+      // counters and screens again before each batch
+      const std::vector<int>& bs = c->batch_start[tb][k];
+      const int nbatch = (int)bs.size() - 1;
+      for (int b = 0; b < nbatch; ++b) {
+        const int u0 = bs[b], nu = bs[b + 1] - bs[b];
+        if (nbatch > 1) {
+          ZeroList zb{};
+          zb.p[0] = (unsigned*)(c->d_counts.p + k);
+          zb.n[0] = 2;
+          zb.p[1] = c->d_nseg.p + k;
+          zb.n[1] = 1;
+          zb.p[2] = c->d_work_q.p + k;
+          zb.n[2] = 1;
+          zb.count = 3;
+          hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, sk, zb);
+          HIPCHK(hipGetLastError());
+        }
+        if (s0 == 0 || nbatch > 1) {
+          hipLaunchKernelGGL(screen_kernel, dim3(grid_for(nu, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p,
+                             kNumClassPairs, c->d_item_start[tb].p, nu, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
+                             c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p, c->nmol, c->d_Nmax.p, c->d_nglob.p, ns,
+                             lo, hi, c->d_pair_nblk.p, c->d_pair_nublk.p, c->d_tasks.p, c->d_counts.p,
+                             c->d_qcounts.p, c->d_segs.p, c->d_nseg.p, k, u0, s0 == 0 ? 1 : 0);
+          HIPCHK(hipGetLastError());
+        }
+        // profiling times the ERI + digestion kernel of each batch (not the screen)
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->profiling) {
+          e0 = c->get_event();
+          e1 = c->get_event();
+          HIPCHK(hipEventRecord(e0, sk));
+        }
+        if (m.flip) {
+          // bra-keyed segments (table 1 only flips where the LDS rows fit)
+          const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
+          const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
+          hipLaunchKernelGGL(fp32 ? ck.jksegb32 : ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
+                             c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
+                             c->d_prims.p, c->d_boys.p, jk);
+        } else if (seg_path) {
+          const int sgrid =
+              (int)std::min<long long>((long long)persistent_grid() * WG / ck.seg_wg, m.seg_cap * ck.nchunk);
+          hipLaunchKernelGGL(fp32 ? ck.jkseg32 : ck.jkseg, dim3(std::max(1, sgrid)), dim3(ck.seg_wg), lds_bytes, sk,
+                             tasks, c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
+                             c->d_prims.p, c->d_boys.p, jk);
+        } else {
+          // (the FP32 round's lists run through the FP64 kernels where the segment path does not fit)
+          hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
+                             c->d_prims.p, c->d_boys.p, jk);
+        }
+        HIPCHK(hipGetLastError());
+        if (c->profiling) {
+          HIPCHK(hipEventRecord(e1, sk));
+          c->prof_events[k].push_back({e0, e1});
+        }
       }
     }
     for (int a = 0; a < cq_ctx::kAux; ++a) {
@@ -1559,6 +1616,34 @@ int cq_get_class_digestion(const cq_ctx_t* c, int k, int* mode) {
   return CQ_OK;
 }
 
+// Largest task list of one class pair, in 8-byte entries (default 2^28, 2 GiB): class pairs
+// with a larger worst case are screened and digested in batches of uniform pairs.  Waits for
+// the context's work, then re-plans the batches and reallocates the task buffers.
+int cq_set_task_budget(cq_ctx_t* c, long long entries) {
+  return guarded(c, [&] {
+    if (entries < 1 || entries > 0x7fffffffLL) throw std::invalid_argument("task budget must be in [1, 2^31 - 1] entries");
+    for (int a = 0; a < cq_ctx::kAux; ++a)
+      if (c->aux[a]) HIPCHK(hipStreamSynchronize(c->aux[a]));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->task_budget = entries;
+    c->d_tasks.release();
+    c->d_segs.release();
+    plan_tasks(c);
+  });
+}
+
+// Task entries allocated over all class pairs, and the largest number of batches any class
+// pair takes (over both digestion tables).
+int cq_get_task_plan(const cq_ctx_t* c, long long* entries, int* max_batches) {
+  if (!c || !entries || !max_batches) return CQ_ERR_INVALID;
+  *entries = c->task_capacity;
+  int mb = 0;
+  for (int t = 0; t < 2; ++t)
+    for (const auto& bs : c->batch_start[t]) mb = std::max(mb, (int)bs.size() - 1);
+  *max_batches = mb;
+  return CQ_OK;
+}
+
 // Number of streams the class-pair kernels are spread over (1 serialises them, for
 // per-kernel profiling; default 4).
 int cq_set_launch_streams(cq_ctx_t* c, int n) {
@@ -1605,6 +1690,10 @@ int cq_get_kernel_times(cq_ctx_t* c, double* ms, long long* launches, int max_n)
 // sum K_ab K_cd, sum K_ab, Cartesian->spherical transform FLOPs, output integrals.
 int cq_get_work_stats(cq_ctx_t* c, double* out, int max_n) {
   return guarded(c, [&] {
+    // the task lists hold the last batch only when a class pair is batched
+    for (int t = 0; t < 2; ++t)
+      for (const auto& bs : c->batch_start[t])
+        if (bs.size() > 2) throw std::invalid_argument("work statistics need every class pair in one batch (raise the task budget)");
     DevBuf<double> acc;
     acc.ensure((size_t)4 * kNumClassPairs);
     HIPCHK(hipMemsetAsync(acc.p, 0, 4 * kNumClassPairs * sizeof(double), c->stream));

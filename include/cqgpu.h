@@ -145,6 +145,18 @@ int cq_set_class_stream(cq_ctx_t* ctx, int k, int stream);
  * over; 1 runs them one after another (for per-kernel timing). */
 int cq_set_launch_streams(cq_ctx_t* ctx, int n);
 
+/* Device memory for the screened task lists (no reference counterpart: the reference screens
+ * and computes quartet by quartet on the host, aointegrals_contract.h:130-146).  The largest
+ * task list of one class pair, in 8-byte entries (1 .. 2^31-1, default 2^28 = 2 GiB); a class
+ * pair whose worst case (every quartet of its pairs passing the screen) is larger is screened
+ * and digested in batches of its uniform pairs, one after another, so memory stays bounded
+ * whatever the system size.  Waits for the context's work and reallocates.  Results agree to
+ * rounding for every budget. */
+int cq_set_task_budget(cq_ctx_t* ctx, long long entries);
+/* Task entries allocated over all class pairs, and the largest number of batches a class pair
+ * takes. */
+int cq_get_task_plan(const cq_ctx_t* ctx, long long* entries, int* max_batches);
+
 /* Profiling (no reference counterpart beyond its wall-clock G[P] timer,
  * include/singleslater/singleslater_fock.h:116-119): HIP events around every class-pair
  * kernel; accumulated milliseconds and launch counts per class pair. */
@@ -152,7 +164,8 @@ int cq_set_profiling(cq_ctx_t* ctx, int on);
 int cq_get_kernel_times(cq_ctx_t* ctx, double* ms, long long* launches, int max_n);
 
 /* Work sums of the last contraction per class pair, 4 doubles each: sum K_ab*K_cd
- * (primitive quartets), sum K_ab, Cartesian->spherical transform FLOPs, output integrals. */
+ * (primitive quartets), sum K_ab, Cartesian->spherical transform FLOPs, output integrals.
+ * CQ_ERR_INVALID when a class pair runs in batches (cq_set_task_budget). */
 int cq_get_work_stats(cq_ctx_t* ctx, double* out, int max_n);
 
 /* Contracted quartets (s1 s2|s3 s4) in the shells' own basis, row-major per quartet,
