@@ -2,7 +2,8 @@
 exceeds the budget is screened and digested in batches of its uniform pairs; J and K, and the
 surviving-quartet counts, stay those of the oracle (aointegrals_contract.h:130-146 visits every
 quartet once whatever the memory), for every digestion mode, for several density slots (the
-lists rebuilt per slot group), in the mixed-precision rounds and in a multi-molecule context."""
+lists rebuilt per slot group), in the mixed-precision rounds, in a multi-molecule context and,
+at the default budget, on a 1536-function cluster."""
 import numpy as np
 import pytest
 
@@ -138,3 +139,36 @@ def test_batched_multi_molecule_context():
     assert n0 == n1
     assert float((J1 - J0).abs().max()) < TOL
     assert float((K1 - K0).abs().max()) < TOL
+
+
+def test_default_budget_batches_large_cluster():
+    """(H2O)64 / cc-pVDZ (1536 functions): the default budget (2^28 entries per class pair)
+    already splits the largest class pairs; a budget of 2^26 splits them further.  On a
+    density that decays with distance like an insulator's, both give the same J and K
+    (1e-10 relative) and the same surviving-quartet count."""
+    import torch
+    mol = M.water_cluster(64, seed=0, box_ang=M.scaled_box(64))
+    bs = B.build_basis("cc-pvdz", mol.Z, mol.coords_bohr)
+    cen = np.concatenate([np.repeat(s.origin[None, :], s.size, 0) for s in bs.shells])
+    r = np.sqrt(((cen[:, None, :] - cen[None, :, :]) ** 2).sum(-1))
+    X = np.random.default_rng(11).uniform(-1, 1, r.shape) * np.exp(-0.7 * r)
+    X = torch.from_numpy(0.5 * (X + X.T)).cuda()
+    res = []
+    for budget in (None, 1 << 26):
+        ao = AOIntegrals(bs)
+        ao.thresholdSchwartz = 1e-12
+        if budget:
+            ao.set_task_budget(budget)
+        nb = ao.task_plan()[1]
+        assert nb > 1
+        J = torch.zeros_like(X)
+        K = torch.zeros_like(X)
+        ao.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+        res.append((J, K, sum(ao.quartet_counts().values()), nb))
+        del ao
+    (J0, K0, n0, b0), (J1, K1, n1, b1) = res
+    assert b1 > b0
+    assert n0 == n1
+    assert float((J1 - J0).abs().max()) < 1e-10 * float(J0.abs().max())
+    assert float((K1 - K0).abs().max()) < 1e-10 * max(1.0, float(K0.abs().max()))
+    assert float((J0 - J0.T).abs().max()) < 1e-10 * float(J0.abs().max())
