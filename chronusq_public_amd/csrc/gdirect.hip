@@ -1625,10 +1625,20 @@ int cq_set_task_budget(cq_ctx_t* c, long long entries) {
     for (int a = 0; a < cq_ctx::kAux; ++a)
       if (c->aux[a]) HIPCHK(hipStreamSynchronize(c->aux[a]));
     HIPCHK(hipStreamSynchronize(c->stream));
+    const long long old = c->task_budget;
     c->task_budget = entries;
     c->d_tasks.release();
     c->d_segs.release();
-    plan_tasks(c);
+    try {
+      plan_tasks(c);
+    } catch (...) {
+      // (out of device memory for the new plan) keep the context usable with the old one
+      c->task_budget = old;
+      c->d_tasks.release();
+      c->d_segs.release();
+      plan_tasks(c);
+      throw;
+    }
   });
 }
 
