@@ -161,3 +161,22 @@ def test_read_guess_from_restart_file(tmp_path):
     assert abs(u2.energy - u1.energy) < 1e-10 and u2.iterations <= 3
     with pytest.raises(RestartError):
         SingleSlater(M.water_test0001(), "6-31g", guess="READ", restart_file=str(tmp_path / "none.bin")).SCF3()
+
+
+def test_scf_densities_are_exactly_symmetric():
+    """The SCF's densities C_occ C_occ^T are symmetric to the last bit at a size where a GEMM
+    rounds (i,j) and (j,i) differently (1536 functions, 160 occupied orbitals), so the
+    contraction never takes its non-symmetric path for them; a UHF SCF's P and Mz too."""
+    import torch
+    from chronusq_public_amd.singleslater import SingleSlater
+    g = torch.Generator(device="cuda").manual_seed(5)
+    Cao = torch.randn(1536, 160, dtype=torch.float64, device="cuda", generator=g)
+    P = SingleSlater._occupied_density(Cao)
+    assert bool((P == P.T).all())
+    assert float((P - Cao @ Cao.T).abs().max()) < 1e-12 * float(P.abs().max())
+    d = json.load(open(os.path.join(GOLD, "energies_uhf.json")))
+    rec = next(iter(d["tests"].values()))
+    sysd = d["systems"][rec["system"]]
+    mol = M.Molecule.from_angstrom(sysd["symbols"], sysd["angstrom"], 0, sysd["mult"])
+    r = SingleSlater(mol, rec["basis"], ref="UHF", tau=1e-14, guess="SAD").SCF3()
+    assert bool((r.P == r.P.T).all()) and bool((r.Mz == r.Mz.T).all())
