@@ -249,15 +249,21 @@ __global__ void norm_kernel(MatList ml, int ld, const ShellMeta* __restrict__ sh
 
 
 // Cartesian density (row-major ncart x ncart): Dc = T^T X T.
-// D_cart = T^T X T for one density; threads below n*n also flag a non-symmetric X.
+// D_cart = T^T X T for one density; threads below n*n also flag a non-symmetric X: one whose
+// X(i,j) and X(j,i) differ by more than 1e-12 of the largest |X| of the call (nglob, from
+// norm_kernel).  Differences below that are GEMM rounding (C C^T need not round (i,j) and
+// (j,i) alike); digesting such an X as symmetric changes G by at most 1e-12 of the call's
+// largest G, while the non-symmetric path would digest the transposed density as well.
 __global__ void cart_density_kernel(const double* __restrict__ X, int n, int ld, const ShellMeta* __restrict__ sh,
                                     const int* __restrict__ cart2shell, int ncart,
                                     const double* __restrict__ tco, const int* __restrict__ tco_off,
+                                    const unsigned long long* __restrict__ nglob_bits,
                                     double* __restrict__ Dc, int* __restrict__ nonsym) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < (long)n * n) {
     const int i = k % n, j = k / n;
-    if (i < j && X[(long)j * ld + i] != X[(long)i * ld + j]) atomicOr(nonsym, 1);
+    const double tol = 1e-12 * __longlong_as_double((long long)*nglob_bits);
+    if (i < j && !(fabs(X[(long)j * ld + i] - X[(long)i * ld + j]) <= tol)) atomicOr(nonsym, 1);
   }
   if (k >= (long)ncart * ncart) return;
   const int a = k / ncart, b = k - (long)a * ncart;
@@ -1160,7 +1166,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
     double* base = c->d_work.p + (size_t)k * 5 * nc2;
     hipLaunchKernelGGL(cart_density_kernel, dim3(grid_for(std::max<long>(nc2, (long)n * n), WG)), dim3(WG), 0, s,
                        slots[k].X, n, ld, c->d_shells.p, c->d_cart2shell.p, c->ncart, c->d_tco.p, c->d_tco_off.p,
-                       base, c->d_flags.p + k);
+                       c->d_nglob.p, base, c->d_flags.p + k);
     HIPCHK(hipGetLastError());
   }
   // digestion variant per class pair (the lds_mode, or the class's own choice below it);

@@ -102,6 +102,30 @@ def test_nonsymmetric_density_matches_reference_scatter(water631):
     assert np.abs(J - J.T).max() > 1e-6   # the reference's Coulomb scatter is order dependent
 
 
+@pytest.mark.parametrize("eps", [2e-13, 1e-9])
+def test_rounding_level_asymmetry_digests_as_symmetric(water631, eps):
+    """X = S + eps*A (A antisymmetric, |X| ~ 1).  Below 1e-12 of max|X| the asymmetry is
+    GEMM-rounding sized and X is digested as symmetric: J comes out symmetric to rounding,
+    where the non-symmetric scatter would leave J - J^T ~ eps |J|.  Above it the reference's
+    non-symmetric scatter runs.  Either way J and K match the oracle, which applies that
+    scatter to X as given."""
+    bs, ao, ob = water631
+    rng = np.random.default_rng(9)
+    S = rng.uniform(-1, 1, (bs.nbasis, bs.nbasis))
+    A = rng.uniform(-1, 1, (bs.nbasis, bs.nbasis))
+    X = np.asfortranarray(0.5 * (S + S.T) + eps * 0.5 * (A - A.T))
+    Qg = ao.computeSchwartz()
+    J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+    (Jo, Ko), _ = _oracle_jk(ob, Qg, [X, X], [0, 1], [1.0, -0.5], 1e-12)
+    assert np.abs(J - Jo).max() < TOL
+    assert np.abs(K - Ko).max() < TOL
+    asym = np.abs(J - J.T).max() / np.abs(J).max()
+    if eps < 1e-12:
+        assert asym < 2e-15, asym
+    else:
+        assert asym > 1e-12, asym
+
+
 def test_uhf_three_matrices_and_accumulate(waterdz):
     """SAD-style UHF call (singleslater_fock.h:49-55): {P,P,Mz}, {J,K,K}; AX accumulates."""
     bs, ao, ob = waterdz
