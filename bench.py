@@ -36,18 +36,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch of each class-pair kernel (FP64 integrals) on this workload: rocprofv3
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors taken from a 256 MiB device copy measured
-# in the same runs (profiles/gdirect_h2o16_ccpvdz_counters.md).  Most of the writes are the
-# FP64 atomics of the J/K digestion.
-TRAFFIC_BYTES = {
-    "(dd|dd)": 28332524, "(dp|dp)": 196870144, "(dp|ds)": 90397737, "(ds|ds)": 34980495,
-    "(pp|ds)": 108538102, "(pp|pp)": 62159585, "(ps|ds)": 189792256, "(ps|ps)": 359451853,
-    "(ss|dp)": 56172216, "(ss|ds)": 103064535, "(ss|pp)": 117230797, "(ss|ps)": 375830610,
-    "(ss|ss)": 156478996, "(dd|dp)": 27294433, "(dd|ds)": 12184453, "(dd|pp)": 18979226,
-    "(dd|ps)": 21390950, "(dd|ss)": 10999562, "(dp|pp)": 91152712, "(dp|ps)": 68398612,
-    "(pp|ps)": 164626432,
-}
+# HBM bytes per launch of the class-pair kernels come from the rocprofv3 counter summary of the
+# same build (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors from a 256 MiB device copy in the
+# same runs); the summary carries the sha256 of the libcqgpu.so it profiled, and the traffic is
+# reported only when that matches the library this run loaded.
+COUNTERS_MD = os.path.join(ROOT, "profiles", "gdirect_h2o16_ccpvdz_counters.md")
 METRIC = "SCF time-to-energy (1e-6 Eh vs FP64) + G[D] quartets/s, (H2O)16 cc-pVDZ"
 
 
@@ -73,6 +66,26 @@ def model_flops(la, lb, lc, ld, sums, nmat):
     A = 37 + 3 * L + 3 * H4(L) + 2 * H(Lab) * H(Lcd) * c(lc) * c(ld)
     Bc = 2 * H(Lab) * c(la) * c(lb) * c(lc) * c(ld)
     return A * kk + Bc * k + ssph + 24.0 * nsph * nmat
+
+
+def profiled_traffic(lib_path):
+    """{class: HBM bytes per launch} from the counter summary, or None (no summary, or one
+    taken from another build of the library)."""
+    import hashlib
+    import re
+    try:
+        text = open(COUNTERS_MD).read()
+    except OSError:
+        return None, "no counter summary"
+    m = re.search(r"<!-- traffic-json: (\{.*?\}) -->", text, re.S)
+    if not m:
+        return None, "counter summary has no traffic record"
+    rec = json.loads(m.group(1))
+    with open(lib_path, "rb") as f:
+        sha = hashlib.sha256(f.read()).hexdigest()
+    if rec.get("lib_sha256") != sha:
+        return None, f"counter summary profiles another build ({rec.get('lib_sha256', '?')[:12]} vs {sha[:12]})"
+    return rec["traffic_bytes"], "rocprofv3 FETCH_SIZE/WRITE_SIZE of this build"
 
 
 def synthetic_density(bs, seed=0):
@@ -123,8 +136,8 @@ def cpu_baseline(bs, D, tau, nthreads, ncalls=3):
 
 
 def run_autotuned(bs, args, dD, dJ, dK, stream, nquart, e0, e1):
-    """G[D] with per-class digestion variants chosen by AOIntegrals.autotune, timed over the
-    same steps as the production configuration."""
+    """G[D] with per-class digestion variants and tiles chosen by AOIntegrals.autotune, timed
+    over the same steps as the production configuration."""
     import torch
     from chronusq_public_amd.aointegrals import AOIntegrals, COULOMB, EXCHANGE
     ao_t = AOIntegrals(bs)
@@ -145,8 +158,7 @@ def run_autotuned(bs, args, dD, dJ, dK, stream, nquart, e0, e1):
     ms_tuned = e0.elapsed_time(e1) / args.steps
     ao_t.close()
     return {"ms_per_step": ms_tuned, "quartets_per_s": nquart / (ms_tuned * 1e-3),
-            "variants": {k: v for k, (v, _) in choice.items()},
-            "variant_ms": {k: t for k, (_, t) in choice.items()}}
+            "choice": {k: {"variant": v, "tile": list(t), "ms": tm[(t, v)]} for k, (v, t, tm) in choice.items()}}
 
 
 def main():
@@ -306,6 +318,9 @@ def main():
     ms = (ctypes.c_double * ncp)()
     nl = (ctypes.c_longlong * ncp)()
     _lib.check(L.cq_get_kernel_times(ao._h, ms, nl, ncp), ao._h)
+    sms = (ctypes.c_double * ncp)()
+    snl = (ctypes.c_longlong * ncp)()
+    _lib.check(L.cq_get_screen_times(ao._h, sms, snl, ncp), ao._h)
     _lib.check(L.cq_set_profiling(ao._h, 0), ao._h)
     _lib.check(L.cq_set_launch_streams(ao._h, 4), ao._h)
     counts = ao.quartet_counts()
@@ -328,11 +343,20 @@ def main():
         per_class.append({"class": "({}{}|{}{})".format(*["spdfg"[x] for x in (la, lb, lc, ld)]),
                           "quartets": int(list(counts.values())[k]), "prim_quartets": ws[4 * k],
                           "model_flop": fl, "kernel_ms": mean_ms,
+                          "screen_ms": sms[k] / snl[k] if snl[k] else 0.0,
                           "tflops": fl / (mean_ms * 1e-3) / 1e12 if mean_ms > 0 else 0.0})
     prim_q = sum(ws[4 * k] for k in range(ncp))
     n_int = sum(ws[4 * k + 3] for k in range(ncp))
     dom = max(per_class, key=lambda r: r["kernel_ms"])
     achieved = dom["model_flop"] / (dom["kernel_ms"] * 1e-3) / 1e12
+    traffic_tab, traffic_src = profiled_traffic(_lib.LIB_PATH)
+    traffic = traffic_tab.get(dom["class"]) if traffic_tab else None
+    # algorithmic bytes of one launch: the Cartesian density read once, J and K written once,
+    # the primitive-pair table (96 B per primitive pair) read once
+    ncart = ctypes.c_int()
+    nprim = ctypes.c_int()
+    _lib.check(L.cq_get_dims(ao._h, None, None, ctypes.byref(ncart), None, ctypes.byref(nprim)), ao._h)
+    alg_bytes = 3 * 8 * ncart.value ** 2 + 96 * nprim.value
     value = nquart / (ms_per_step * 1e-3)
     out = {
         "metric": METRIC, "value": value, "unit": "quartets/s", "n_gpus": 1,
@@ -345,11 +369,16 @@ def main():
                    "prim_quartets_per_s": prim_q / (ms_per_step * 1e-3),
                    "integrals_per_s": n_int / (ms_per_step * 1e-3),
                    "serial_kernels_ms": sum(r["kernel_ms"] for r in per_class),
+                   "serial_screen_ms": sum(r["screen_ms"] for r in per_class),
                    "all_kernels_tflops": total_flops / (sum(r["kernel_ms"] for r in per_class) * 1e-3) / 1e12,
                    "step_tflops": total_flops / (ms_per_step * 1e-3) / 1e12},
-        "roofline": {"bound": "mfma", "pipe": "FP64 VALU (MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s)", "kernel": "eri_jk" + dom["class"], "achieved": achieved,
+        "roofline": {"bound": "fp64-valu",
+                     "pipe": "FP64 vector ALU (no ERI kernel issues MFMA; the FP64 vector peak is 78.6 TFLOP/s)",
+                     "kernel": "eri_grp" + dom["class"], "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": TRAFFIC_BYTES.get(dom["class"])},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": alg_bytes,
+                     "traffic_over_algorithmic": traffic / alg_bytes if traffic else None},
         "autotuned": autotuned,
         "scf": scf,
         "batch": batch,

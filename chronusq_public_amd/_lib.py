@@ -26,9 +26,9 @@ EXPORTED = [
     "cq_get_schwarz_threshold", "cq_set_precision_mode", "cq_get_dims", "cq_get_schwarz",
     "cq_contract_direct", "cq_contract_direct_device", "cq_get_quartet_counts",
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
-    "cq_get_kernel_times", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_set_launch_streams", "cq_set_lds_digestion",
+    "cq_get_kernel_times", "cq_get_screen_times", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_set_launch_streams", "cq_set_lds_digestion",
     "cq_solid_harmonics", "cq_set_class_digestion", "cq_get_class_digestion",
-    "cq_set_class_stream", "cq_set_task_budget", "cq_get_task_plan",
+    "cq_set_class_stream", "cq_set_task_budget", "cq_get_task_plan", "cq_set_class_tile", "cq_get_class_tile",
     # learned initial guess (include/cqgpu_guess.h)
     "cq_mlp_last_error", "cq_mlp_gemm_bf16", "cq_mlp_relu_backward", "cq_mlp_column_sum", "cq_mlp_to_bf16",
     "cq_mlp_mse", "cq_mlp_adam",
@@ -81,6 +81,7 @@ def load():
     L.cq_one_electron.argtypes = [vp, ctypes.c_int, dp, dp, vp, vp, vp, ctypes.c_int, vp]
     L.cq_one_electron_batch.argtypes = [vp, ctypes.c_int, dp, dp, ip, vp, vp, vp, ctypes.c_int, vp]
     L.cq_get_kernel_times.argtypes = [vp, dp, P(ctypes.c_longlong), ctypes.c_int]
+    L.cq_get_screen_times.argtypes = [vp, dp, P(ctypes.c_longlong), ctypes.c_int]
     L.cq_get_work_stats.argtypes = [vp, dp, ctypes.c_int]
     L.cq_solid_harmonics.argtypes = [ctypes.c_int, dp]
     L.cq_set_class_digestion.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -88,6 +89,8 @@ def load():
     L.cq_set_class_stream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.cq_set_task_budget.argtypes = [vp, ctypes.c_longlong]
     L.cq_get_task_plan.argtypes = [vp, P(ctypes.c_longlong), ip]
+    L.cq_set_class_tile.argtypes = [vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int]
+    L.cq_get_class_tile.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, ip, ip, P(ctypes.c_longlong)]
     ci, cl, cf = ctypes.c_int, ctypes.c_long, ctypes.c_float
     L.cq_mlp_last_error.argtypes = []
     L.cq_mlp_last_error.restype = ctypes.c_char_p

@@ -20,7 +20,7 @@ the GPU; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -115,46 +115,60 @@ class AOIntegrals:
             raise ValueError(f"unknown precision mode {mode!r}")
         check(_lib.load().cq_set_precision_mode(self._h, code, float(threshold)), self._h)
 
-    def autotune(self, X, reps: int = 3, variants: Sequence[int] = (0, 1, 2), balance_streams: bool = True) -> dict:
-        """Pick, per class pair, the fastest J/K digestion variant for the RHF contraction of
-        the device density X (0: FP64 atomics into HBM, 1: ket-keyed LDS segments, 2: also
-        bra-keyed segments where the class pair allows them).  Every variant runs `reps`
-        times with the class kernels serialised and timed by HIP events
-        (cq_get_kernel_times); the fastest is set per class pair (cq_set_class_digestion).
-        With balance_streams the class pairs are then placed on the four streams longest
-        first, each on the stream with the least kernel time so far (cq_set_class_stream).
-        Returns {class name: (chosen variant, {variant: mean ms})}.  No reference
-        counterpart."""
+    DEFAULT_TILES = ((0, 0), (1, 256), (1, 512), (1, 1024), (2, 512), (4, 512), (4, 1024))
+
+    def autotune(self, X, reps: int = 3, variants: Sequence[int] = (0, 1, 2),
+                 tiles: Optional[Sequence[Tuple[int, int]]] = None, balance_streams: bool = True) -> dict:
+        """Pick, per class pair, the fastest J/K digestion variant and tile for the RHF
+        contraction of the device density X.  Variants: 0 K updates as FP64 atomics into HBM,
+        1 ket-keyed groups with K columns in LDS, 2 also bra-keyed groups where the class pair
+        allows them.  Tiles (max_group, union_bound) as in set_class_tile, (0, 0) being the
+        built-in default.  Every (tile, variant) runs `reps` times with the class kernels
+        serialised and timed by HIP events (cq_get_kernel_times + cq_get_screen_times: the
+        screening is part of the cost); the fastest is set per class pair
+        (cq_set_class_tile, cq_set_class_digestion).  With balance_streams the class pairs are
+        then placed on the four streams longest first, each on the stream with the least
+        kernel time so far (cq_set_class_stream).  Returns {class name: (variant, tile,
+        {(tile, variant): mean ms})}.  No reference counterpart."""
         import torch
         L = _lib.load()
         nc = L.cq_num_class_pairs()
+        tiles = list(tiles) if tiles is not None else list(self.DEFAULT_TILES)
         J = torch.zeros_like(X)
         K = torch.zeros_like(X)
         times = {}
         check(L.cq_set_launch_streams(self._h, 1), self._h)
         try:
-            for v in variants:
+            for tile in tiles:
                 for k in range(nc):
-                    check(L.cq_set_class_digestion(self._h, k, int(v)), self._h)
-                self.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
-                check(L.cq_set_profiling(self._h, 1), self._h)
-                for _ in range(reps):
+                    check(L.cq_set_class_tile(self._h, k, 0, int(tile[0]), int(tile[1])), self._h)
+                for v in variants:
+                    for k in range(nc):
+                        check(L.cq_set_class_digestion(self._h, k, int(v)), self._h)
                     self.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
-                ms = (ctypes.c_double * nc)()
-                nl = (ctypes.c_longlong * nc)()
-                check(L.cq_get_kernel_times(self._h, ms, nl, nc), self._h)
-                check(L.cq_set_profiling(self._h, 0), self._h)
-                times[int(v)] = [ms[k] / nl[k] if nl[k] else 0.0 for k in range(nc)]
+                    check(L.cq_set_profiling(self._h, 1), self._h)
+                    for _ in range(reps):
+                        self.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+                    ms = (ctypes.c_double * nc)()
+                    nl = (ctypes.c_longlong * nc)()
+                    sms = (ctypes.c_double * nc)()
+                    snl = (ctypes.c_longlong * nc)()
+                    check(L.cq_get_kernel_times(self._h, ms, nl, nc), self._h)
+                    check(L.cq_get_screen_times(self._h, sms, snl, nc), self._h)
+                    check(L.cq_set_profiling(self._h, 0), self._h)
+                    times[(tuple(tile), int(v))] = [(ms[k] / nl[k] if nl[k] else 0.0) +
+                                                    (sms[k] / snl[k] if snl[k] else 0.0) for k in range(nc)]
         finally:
             check(L.cq_set_launch_streams(self._h, 4), self._h)
         names = list(self.quartet_counts().keys())
         out = {}
         chosen_ms = []
         for k in range(nc):
-            best = min(times, key=lambda v: times[v][k])
-            check(L.cq_set_class_digestion(self._h, k, best), self._h)
-            out[names[k]] = (best, {v: times[v][k] for v in times})
-            chosen_ms.append(times[best][k])
+            (tile, v) = min(times, key=lambda tv: times[tv][k])
+            check(L.cq_set_class_tile(self._h, k, 0, int(tile[0]), int(tile[1])), self._h)
+            check(L.cq_set_class_digestion(self._h, k, v), self._h)
+            out[names[k]] = (v, tile, {tv: times[tv][k] for tv in times})
+            chosen_ms.append(times[(tile, v)][k])
         load = [0.0] * 4
         for k in sorted(range(nc), key=lambda k: -chosen_ms[k]):
             s = min(range(4), key=lambda i: load[i]) if balance_streams else -1
@@ -162,6 +176,20 @@ class AOIntegrals:
             if s >= 0:
                 load[s] += chosen_ms[k]
         return out
+
+    def set_class_tile(self, k: int, max_group: int = 0, union_bound: int = 0, lds_bytes: int = 0) -> None:
+        """Tile of class pair k's digestion kernels (cq_set_class_tile): groups of up to
+        max_group uniform pairs, task regions of up to union_bound swept pairs, a dynamic LDS
+        budget; zeros select the defaults."""
+        check(_lib.load().cq_set_class_tile(self._h, int(k), int(lds_bytes), int(max_group), int(union_bound)), self._h)
+
+    def class_tile(self, k: int, variant: int = 1) -> Tuple[int, int, int, int]:
+        """(max group, union bound, groups, kernel LDS bytes) planned for class pair k."""
+        mg, ub, ng = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lds = ctypes.c_longlong()
+        check(_lib.load().cq_get_class_tile(self._h, int(k), int(variant), ctypes.byref(mg), ctypes.byref(ub),
+                                            ctypes.byref(ng), ctypes.byref(lds)), self._h)
+        return mg.value, ub.value, ng.value, lds.value
 
     def class_digestion(self) -> List[int]:
         L = _lib.load()

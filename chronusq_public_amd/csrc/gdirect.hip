@@ -44,13 +44,14 @@
 #include "_gen/eri_registry.inc"
 
 static ClassKernels kClassKernels[CQ_NUM_CLASS_PAIRS];
-// dynamic LDS of the segment kernels: what the 64 KiB per workgroup leaves beside the static
-// LDS (Boys table, 25992 B, and the work-item slot)
+#define MAX_MATS 8   // matrices of one contraction
+// dynamic LDS of the group kernels: what the 64 KiB per workgroup leaves beside the static
+// LDS (Boys table, 25992 B, the work-item slot and the members' column offsets)
 static const size_t kStaticLds = BOYS_NGRID * BOYS_LDS_W * sizeof(double) + 256;
 static const size_t kMaxDynLds = 64 * 1024 - kStaticLds;
 
-// Dynamic LDS the segment kernels may use on this device: the device's per-workgroup limit
-// (160 KiB on gfx950) less their static LDS, once every segment kernel has been allowed that
+// Dynamic LDS the group kernels may use on this device: the device's per-workgroup limit
+// (160 KiB on gfx950) less their static LDS, once every group kernel has been allowed that
 // much (hipFuncSetAttribute); 64 KiB less the static LDS if that fails.  Computed once.
 static size_t device_dyn_lds() {
   static size_t v = 0;
@@ -65,7 +66,7 @@ static size_t device_dyn_lds() {
     const size_t want = (size_t)lim - kStaticLds;
     for (int k = 0; k < CQ_NUM_CLASS_PAIRS; ++k) {
       const ClassKernels& ck = kClassKernels[k];
-      for (SegKernel f : {ck.jkseg, ck.jksegb, ck.jkseg32, ck.jksegb32}) {
+      for (GroupKernel f : {ck.grp[0][0], ck.grp[0][1], ck.grp[0][2], ck.grp[1][0], ck.grp[1][1], ck.grp[1][2]}) {
         if (!f) continue;
         if (hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want) != hipSuccess) {
           (void)hipGetLastError();
@@ -81,127 +82,212 @@ static const int kNumClassPairs = CQ_NUM_CLASS_PAIRS;
 static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 
 // ---------------------------------------------------------------------------------------
-// Screening: one wavefront per (class pair, bra pair); compacted task lists.
+// Screening: one wavefront per uniform group; task regions (layout in gdirect_kernels.h).
 // ---------------------------------------------------------------------------------------
 struct CPMeta {
   int bra_start, nbra, ket_start, nket, same, bra_class, ket_class;
-  int flip;   // 1: segments are keyed on the bra pair (bra-major task lists)
+  int flip;        // 1: the uniform side is the bra class (bra-keyed groups)
+  int grp_off;     // first group of this class pair in its table's group array
+  int ngrp;
+  int ub;          // union bound of a task region (swept pairs), 64 <= ub <= UNION_MAX
+  int mmax;        // largest group
   long long task_off;
-  long long task_cap;   // task entries (primitive blocks of the swept pairs included)
+  long long task_cap;   // task entries of the largest batch of groups
   long long seg_off;
   long long seg_cap;
 };
 
-// One wavefront per (class pair, uniform pair); the uniform pair is the ket (or the bra when
-// flip is set) and the lanes sweep the other class's pairs (sorted by Schwarz bound, so the
-// sweep stops once no pair can pass).  Survivors are staged in a per-wavefront LDS buffer and
-// flushed with one atomic per buffer, so task lists come out grouped by the uniform pair: the
-// lanes of an ERI wavefront then share it.  Tasks are stored as (bra, ket) either way.
-#define SCREEN_BUF 1024
+#define UNION_MAX 1024
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Each wavefront takes one group of the class pair (groups item0 .. item0 + nitem - 1 of it: one
+// batch) and sweeps the other class's pairs of the group's molecule, sorted by Schwarz bound
+// inside each primitive-count group, so a sweep stops once no member can pass.  Each lane tests
+// its swept pair against every member with the reference's test
+//   max(N(s1,s4), N(s2,s4), N(s3,s4), N(s1,s3), N(s2,s3), N(s1,s2)) * Q(s1,s2) * Q(s3,s4) >= tau
+// in the canonical order of the quartet (aointegrals_contract.h:111-131); the members it passes
+// form a bit mask.  Swept pairs with a non-empty mask join the union (LDS); a full union (or the
+// end of the sweep) is written out as one task region plus one segment per primitive block of
+// the members.  [tau, tau_hi) is the bound window of this launch (mixed precision).
 __global__ void __launch_bounds__(WG) screen_kernel(
-    const CPMeta* __restrict__ cps, int ncp, const int* __restrict__ item_start, int nitem,
-    const int2* __restrict__ s12, const double* __restrict__ pairQ, const int* __restrict__ grp,
-    const int* __restrict__ cls_grp_off, const int* __restrict__ pair_mol, const int* __restrict__ cmg, int nmol,
-    const double* __restrict__ Nmax,
+    const CPMeta* __restrict__ cps, int cp, int item0, int nitem, const GroupRec* __restrict__ groups,
+    const int* __restrict__ uorder, const int2* __restrict__ s12, const double* __restrict__ pairQ,
+    const int* __restrict__ grp, const int* __restrict__ cls_grp_off, const int* __restrict__ pair_mol,
+    const int* __restrict__ cmg, int nmol, const double* __restrict__ Nmax, const double* __restrict__ Npair,
     const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, double tau_hi,
-    const unsigned char* __restrict__ pair_nblk, const unsigned char* __restrict__ pair_nublk,
-    uint2* __restrict__ tasks,
-    unsigned long long* __restrict__ counts, unsigned long long* __restrict__ qcounts,
-    Segment* __restrict__ segs, unsigned* __restrict__ nseg, int cp_fixed, int item0, int count_quartets) {
-  __shared__ uint2 buf[WG / 64][SCREEN_BUF];
+    const unsigned char* __restrict__ pair_nblk, uint2* __restrict__ tasks, unsigned long long* __restrict__ counts,
+    unsigned long long* __restrict__ qcounts, Segment* __restrict__ segs, unsigned* __restrict__ nseg,
+    int count_quartets) {
+  __shared__ int ucode[WG / 64][UNION_MAX];
+  __shared__ unsigned short umask[WG / 64][UNION_MAX];
+  __shared__ double mQ[WG / 64][MAX_GROUP], mN[WG / 64][MAX_GROUP];
+  __shared__ int ms1[WG / 64][MAX_GROUP], ms2[WG / 64][MAX_GROUP], mloc[WG / 64][MAX_GROUP];
+  __shared__ unsigned mcnt[WG / 64][MAX_GROUP];
   const int wid = threadIdx.x >> 6;
   const int wave = blockIdx.x * (WG / 64) + wid;
   const int lane = threadIdx.x & 63;
   if (wave >= nitem) return;
-  // cp_fixed >= 0: this launch screens that class pair only, its uniform pairs
-  // item0 .. item0 + nitem - 1 (one batch)
-  int cp = cp_fixed >= 0 ? cp_fixed : 0;
-  if (cp_fixed < 0)
-    while (cp + 1 < ncp && item_start[cp + 1] <= wave) ++cp;
   const CPMeta m = cps[cp];
-  const int ket_local = cp_fixed >= 0 ? item0 + wave : wave - item_start[cp];
-  const int K = (m.flip ? m.bra_start : m.ket_start) + ket_local;   // the uniform pair
-  const int sw_start = m.flip ? m.ket_start : m.bra_start;           // the swept class
+  const int g = m.grp_off + item0 + wave;
+  const GroupRec gr = groups[g];
+  const int n = gr.n, ublk = gr.ublk;
+  const int uni_start = m.flip ? m.bra_start : m.ket_start;
+  const int sw_start = m.flip ? m.ket_start : m.bra_start;
   const int nsw = m.flip ? m.nket : m.nbra;
   const int sw_class = m.flip ? m.ket_class : m.bra_class;
-  const int i_begin = m.same ? ket_local : 0;
-  const double Qk = pairQ[K];
-  const int2 sk = s12[K];
-  const long canonK = (long)sk.x * (sk.x + 1) / 2 + sk.y;
+  if (lane < n) {
+    const int K = uorder[gr.first + lane];
+    const int2 sk = s12[K];
+    mQ[wid][lane] = pairQ[K];
+    mN[wid][lane] = Npair[K];
+    ms1[wid][lane] = sk.x;
+    ms2[wid][lane] = sk.y;
+    mloc[wid][lane] = K - uni_start;
+  }
+  wave_sync_lds();
+  double Qkmax = 0.0;
+  int lmin = 0x7fffffff;
+  for (int mm = 0; mm < n; ++mm) {
+    Qkmax = fmax(Qkmax, mQ[wid][mm]);
+    lmin = min(lmin, mloc[wid][mm]);
+  }
+  const int i_begin = m.same ? lmin : 0;
   const double nglob = __longlong_as_double((long long)*nglob_bits);
-  uint2* wbuf = buf[wid];
-  int nb = 0;
+  // exactly symmetric norms: the six norms of a quartet are the two pairs' own norms and the
+  // four cross norms N(k, p), k a shell of the uniform pair, p of the swept one -- read along
+  // the uniform pair's rows (cache-local for the whole wavefront)
+  const bool symN = nglob_bits[1 + MAX_MATS] == 0ull;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  uint2* const tout = tasks + m.task_off;
+  int U = 0;
   unsigned long long nq = 0;
-  // a uniform pair with more than UNI_BLOCK primitive pairs gets one segment per block
-  const int ublk = pair_nublk[K];
   auto flush = [&]() {
     wave_sync_lds();
+    // entries per member: one per primitive block of each union pair it passed
+    unsigned cl[MAX_GROUP];
+#pragma unroll
+    for (int mm = 0; mm < MAX_GROUP; ++mm) cl[mm] = 0u;
+    for (int u = lane; u < U; u += 64) {
+      const unsigned nb = pair_nblk[ucode[wid][u]];
+      const unsigned msk = umask[wid][u];
+#pragma unroll
+      for (int mm = 0; mm < MAX_GROUP; ++mm) cl[mm] += ((msk >> mm) & 1u) * nb;
+    }
+    unsigned tot = 0;
+#pragma unroll
+    for (int mm = 0; mm < MAX_GROUP; ++mm) {
+      if (mm >= n) break;
+      const unsigned c = wave_sum(cl[mm]);
+      if (lane == 0) mcnt[wid][mm] = c;
+      tot += c;
+    }
+    wave_sync_lds();
+    const unsigned hdr = 1u + (unsigned)n;
+    const unsigned total = hdr + tot;
     unsigned long long base = 0;
     unsigned si = 0;
     if (lane == 0) {
-      base = atomicAdd(&counts[cp], (unsigned long long)nb);
+      base = atomicAdd(&counts[cp], (unsigned long long)total);
       si = atomicAdd(&nseg[cp], (unsigned)ublk);
     }
     base = __shfl(base, 0);
     si = __shfl(si, 0);
-    if (lane < ublk) segs[m.seg_off + si + lane] = Segment{(int)base, nb, K, ublk > 1 ? lane + 1 : 0};
-    for (int k = lane; k < nb; k += 64) tasks[m.task_off + base + k] = wbuf[k];
-    wave_sync_lds();
-    nb = 0;
-  };
-  // the swept pairs of the uniform pair's own molecule (one molecule: the whole class)
-  const int g_hi = cls_grp_off[sw_class + 1];
-  const int mK = pair_mol[K];
-  const int gm_lo = cmg[sw_class * (nmol + 1) + mK], gm_hi = cmg[sw_class * (nmol + 1) + mK + 1];
-  for (int g = gm_lo; g < gm_hi; ++g) {
-  const int grp_end = (g + 1 < g_hi) ? grp[g + 1] : nsw;
-  if (grp_end <= i_begin) continue;
-  for (int i0 = max(grp[g], i_begin); i0 < grp_end; i0 += 64) {
-    const double Qi0 = pairQ[sw_start + i0];
-    if (nglob * Qk * Qi0 * (1.0 + 1e-12) < tau) break;
-    const int i = i0 + lane;
-    bool pass = false;
-    if (i < grp_end) {
-      const int P = sw_start + i;
-      const double Qp = pairQ[P];
-      const int2 sp = s12[P];
-      const long canonP = (long)sp.x * (sp.x + 1) / 2 + sp.y;
-      int s1, s2, s3, s4;
-      double Q12, Q34;
-      if (canonP >= canonK) { s1 = sp.x; s2 = sp.y; s3 = sk.x; s4 = sk.y; Q12 = Qp; Q34 = Qk; }
-      else                  { s1 = sk.x; s2 = sk.y; s3 = sp.x; s4 = sp.y; Q12 = Qk; Q34 = Qp; }
-      const double n14 = Nmax[s1 * nshell + s4], n24 = Nmax[s2 * nshell + s4];
-      const double n34 = Nmax[s3 * nshell + s4], n13 = Nmax[s1 * nshell + s3];
-      const double n23 = Nmax[s2 * nshell + s3], n12 = Nmax[s1 * nshell + s2];
-      const double mx = fmax(n14, fmax(n24, fmax(n34, fmax(n13, fmax(n23, n12)))));
-      // the reference's test, and the upper end of this launch's bound window
-      const double bound = __dmul_rn(__dmul_rn(mx, Q12), Q34);
-      pass = !(bound < tau) && bound < tau_hi;
+    if (lane < ublk) segs[m.seg_off + si + lane] = Segment{(int)base, (int)total, g, ublk > 1 ? lane + 1 : 0};
+    uint2* out = tout + base;
+    if (lane == 0) out[0] = make_uint2((unsigned)n, (unsigned)U);
+    if (lane < n) {
+      unsigned off = hdr;
+      for (int mm = 0; mm < lane; ++mm) off += mcnt[wid][mm];
+      out[1 + lane] = make_uint2(mcnt[wid][lane], off);
     }
-    nq += __popcll(__ballot(pass));
-    // one task entry per primitive block of the swept pair
-    const int pblk = pass ? pair_nblk[sw_start + i] : 0;
-    for (int j = 0;; ++j) {
-      const bool e = j < pblk;
-      const unsigned long long mask = __ballot(e);
-      if (mask == 0ull) break;
-      const int cnt = __popcll(mask);
-      if (nb + cnt > SCREEN_BUF) flush();
-      if (e) {
-        const unsigned P = (unsigned)(sw_start + i) | (pblk > 1 ? (unsigned)(j + 1) << 24 : 0u);
-        wbuf[nb + __popcll(mask & ((1ull << lane) - 1ull))] = m.flip ? make_uint2((unsigned)K, P) : make_uint2(P, (unsigned)K);
+    unsigned run = hdr;
+    for (int mm = 0; mm < n; ++mm) {
+      for (int u0 = 0; u0 < U; u0 += 64) {
+        const int u = u0 + lane;
+        const unsigned code = u < U ? (unsigned)ucode[wid][u] : 0u;
+        const int nb = (u < U && ((umask[wid][u] >> mm) & 1u)) ? (int)pair_nblk[code] : 0;
+        for (int j = 0;; ++j) {
+          const bool e = j < nb;
+          const unsigned long long bal = __ballot(e);
+          if (bal == 0ull) break;
+          if (e) out[run + __popcll(bal & lt)] = make_uint2(code | (nb > 1 ? (unsigned)(j + 1) << 24 : 0u), (unsigned)u);
+          run += (unsigned)__popcll(bal);
+        }
       }
-      nb += cnt;
+    }
+    wave_sync_lds();
+    U = 0;
+  };
+  // the swept pairs of the group's own molecule (one molecule: the whole class)
+  const int g_hi = cls_grp_off[sw_class + 1];
+  const int mK = pair_mol[uorder[gr.first]];
+  const int gm_lo = cmg[sw_class * (nmol + 1) + mK], gm_hi = cmg[sw_class * (nmol + 1) + mK + 1];
+  for (int gq = gm_lo; gq < gm_hi; ++gq) {
+    const int grp_end = (gq + 1 < g_hi) ? grp[gq + 1] : nsw;
+    if (grp_end <= i_begin) continue;
+    for (int i0 = max(grp[gq], i_begin); i0 < grp_end; i0 += 64) {
+      const double Qi0 = pairQ[sw_start + i0];
+      if (nglob * Qkmax * Qi0 * (1.0 + 1e-12) < tau) break;
+      const int i = i0 + lane;
+      unsigned mask = 0u;
+      if (i < grp_end) {
+        const int P = sw_start + i;
+        const double Qp = pairQ[P];
+        const int2 sp = s12[P];
+        const double Np = symN ? Npair[P] : 0.0;
+        const long canonP = (long)sp.x * (sp.x + 1) / 2 + sp.y;
+        for (int mm = 0; mm < n; ++mm) {
+          const double Qk = mQ[wid][mm];
+          if (nglob * Qk * Qi0 * (1.0 + 1e-12) < tau) continue;   // no swept pair from here passes mm
+          if (m.same && i < mloc[wid][mm]) continue;               // each unordered pair once
+          const int kx = ms1[wid][mm], ky = ms2[wid][mm];
+          const long canonK = (long)kx * (kx + 1) / 2 + ky;
+          const bool pfirst = canonP >= canonK;
+          const double Q12 = pfirst ? Qp : Qk, Q34 = pfirst ? Qk : Qp;
+          double mx;
+          if (symN) {
+            const double* rx = Nmax + (long)kx * nshell;
+            const double* ry = Nmax + (long)ky * nshell;
+            mx = fmax(fmax(Np, mN[wid][mm]), fmax(fmax(rx[sp.x], rx[sp.y]), fmax(ry[sp.x], ry[sp.y])));
+          } else {
+            int s1, s2, s3, s4;
+            if (pfirst) { s1 = sp.x; s2 = sp.y; s3 = kx; s4 = ky; }
+            else        { s1 = kx; s2 = ky; s3 = sp.x; s4 = sp.y; }
+            const double n14 = Nmax[s1 * nshell + s4], n24 = Nmax[s2 * nshell + s4];
+            const double n34 = Nmax[s3 * nshell + s4], n13 = Nmax[s1 * nshell + s3];
+            const double n23 = Nmax[s2 * nshell + s3], n12 = Nmax[s1 * nshell + s2];
+            mx = fmax(n14, fmax(n24, fmax(n34, fmax(n13, fmax(n23, n12)))));
+          }
+          // the reference's test, and the upper end of this launch's bound window
+          const double bound = __dmul_rn(__dmul_rn(mx, Q12), Q34);
+          if (!(bound < tau) && bound < tau_hi) mask |= 1u << mm;
+        }
+      }
+      nq += (unsigned long long)__popc(mask);
+      const bool has = mask != 0u;
+      const unsigned long long bal = __ballot(has);
+      if (has) {
+        const int slot = U + __popcll(bal & lt);
+        ucode[wid][slot] = sw_start + i;
+        umask[wid][slot] = (unsigned short)mask;
+      }
+      U += __popcll(bal);
+      if (U > m.ub - 64) flush();
     }
   }
-  }
-  if (nb > 0) flush();
-  // (a list screened again for a later slot group is not counted twice)
+  if (U > 0) flush();
+  // (a list screened again for a later slot or batch is not counted twice)
+  nq = wave_sum(nq);
   if (lane == 0 && nq > 0 && count_quartets) atomicAdd(&qcounts[cp], nq);
 }
 
@@ -221,48 +307,78 @@ __device__ __forceinline__ double tcoef(const ShellMeta& s, int mu, int a, const
   return tco[tco_off[s.l] + mu * nc + a];
 }
 
-#define MAX_MATS 8
 struct MatList {
   const double* X[MAX_MATS];
   int n;
 };
 
 // Block infinity norms of every (s,t) shell block, maximised over the matrices; also the
-// global maximum (as the bit pattern of a non-negative double, so atomicMax orders it).
+// largest |X| of the call (nglob_bits[0]) and of each matrix m (nglob_bits[1 + m]), as the bit
+// patterns of non-negative doubles, so that atomicMax orders them.  Reduced across the
+// wavefront first: one atomic per wavefront and value.
+__device__ __forceinline__ void wave_max_bits(unsigned long long* p, double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+// nglob_bits[1 + MAX_MATS] is set when the norm matrix is not exactly symmetric (some
+// N(s,t) != N(t,s)): the screening then evaluates the reference's six norms in its order.
 __global__ void norm_kernel(MatList ml, int ld, const ShellMeta* __restrict__ sh, int nshell,
                             double* __restrict__ Nmax, unsigned long long* __restrict__ nglob_bits) {
   const int st = blockIdx.x * blockDim.x + threadIdx.x;
-  if (st >= nshell * nshell) return;
-  const int s = st / nshell, t = st - s * nshell;
-  const ShellMeta A = sh[s], B = sh[t];
-  const int na = A.pure ? 2 * A.l + 1 : (A.l + 1) * (A.l + 2) / 2;
-  const int nb = B.pure ? 2 * B.l + 1 : (B.l + 1) * (B.l + 2) / 2;
-  double mx = 0.0;
+  const bool in = st < nshell * nshell;
+  const int s = in ? st / nshell : 0, t = in ? st - s * nshell : 0;
+  double mx = 0.0, mxT = 0.0;
   for (int m = 0; m < ml.n; ++m) {
-    const double* X = ml.X[m];
-    for (int j = 0; j < nb; ++j)
-      for (int i = 0; i < na; ++i) mx = fmax(mx, fabs(X[(long)(B.sph_off + j) * ld + A.sph_off + i]));
+    double mxm = 0.0;
+    if (in) {
+      const ShellMeta A = sh[s], B = sh[t];
+      const int na = A.pure ? 2 * A.l + 1 : (A.l + 1) * (A.l + 2) / 2;
+      const int nb = B.pure ? 2 * B.l + 1 : (B.l + 1) * (B.l + 2) / 2;
+      const double* X = ml.X[m];
+      for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < na; ++i) mxm = fmax(mxm, fabs(X[(long)(B.sph_off + j) * ld + A.sph_off + i]));
+      if (s < t) {   // the transposed block, N(t, s)
+        for (int j = 0; j < nb; ++j)
+          for (int i = 0; i < na; ++i) mxT = fmax(mxT, fabs(X[(long)(A.sph_off + i) * ld + B.sph_off + j]));
+      }
+    }
+    mx = fmax(mx, mxm);
+    wave_max_bits(&nglob_bits[1 + m], mxm);
   }
-  Nmax[st] = mx;
-  atomicMax(nglob_bits, (unsigned long long)__double_as_longlong(mx));
+  if (in) Nmax[st] = mx;
+  wave_max_bits(&nglob_bits[0], mx);
+  if (in && s < t && mx != mxT) atomicOr(&nglob_bits[1 + MAX_MATS], 1ull);
+}
+
+// Npair[P] = N(s1, s2) of every shell pair (the pair's own block norm, read coalesced by the
+// screening).
+__global__ void pair_norm_kernel(const int2* __restrict__ s12, int npair, const double* __restrict__ Nmax,
+                                 int nshell, double* __restrict__ Npair) {
+  const int P = blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= npair) return;
+  const int2 sp = s12[P];
+  Npair[P] = Nmax[sp.x * nshell + sp.y];
 }
 
 
 // Cartesian density (row-major ncart x ncart): Dc = T^T X T.
 // D_cart = T^T X T for one density; threads below n*n also flag a non-symmetric X: one whose
-// X(i,j) and X(j,i) differ by more than 1e-12 of the largest |X| of the call (nglob, from
+// X(i,j) and X(j,i) differ by more than 1e-12 of this X's own largest |X| (xmax_bits, from
 // norm_kernel).  Differences below that are GEMM rounding (C C^T need not round (i,j) and
-// (j,i) alike); digesting such an X as symmetric changes G by at most 1e-12 of the call's
-// largest G, while the non-symmetric path would digest the transposed density as well.
+// (j,i) alike).  Digesting such an X as symmetric replaces it by (X + X^T)/2, an entrywise
+// change of at most 5e-13 max|X|, so each G element moves by at most 5e-13 max|X| times the
+// sum of |ERI| over the quartets that element collects; the non-symmetric path would digest
+// the transposed density as well (about four times the work).
 __global__ void cart_density_kernel(const double* __restrict__ X, int n, int ld, const ShellMeta* __restrict__ sh,
                                     const int* __restrict__ cart2shell, int ncart,
                                     const double* __restrict__ tco, const int* __restrict__ tco_off,
-                                    const unsigned long long* __restrict__ nglob_bits,
+                                    const unsigned long long* __restrict__ xmax_bits,
                                     double* __restrict__ Dc, int* __restrict__ nonsym) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < (long)n * n) {
     const int i = k % n, j = k / n;
-    const double tol = 1e-12 * __longlong_as_double((long long)*nglob_bits);
+    const double tol = 1e-12 * __longlong_as_double((long long)*xmax_bits);
     if (i < j && !(fabs(X[(long)j * ld + i] - X[(long)i * ld + j]) <= tol)) atomicOr(nonsym, 1);
   }
   if (k >= (long)ncart * ncart) return;
@@ -402,51 +518,71 @@ __global__ void schwarz_kernel(const double* __restrict__ blocks, const int* __r
   pairQ[P] = sqrt(mx);
 }
 
-// Work sums over one class pair's task list, for the algorithmic FLOP model:
-// acc[0] += K_ab K_cd (primitive quartets), acc[1] += K_ab, acc[2] += sequential
-// Cartesian->spherical transform cost, acc[3] += output integrals in the shells' basis.
-__global__ void work_stats_kernel(const uint2* __restrict__ tasks, const unsigned long long* __restrict__ count_ptr,
-                                  const PairInfo* __restrict__ pairs, const ShellMeta* __restrict__ sh,
-                                  double* __restrict__ acc) {
-  const long count = (long)*count_ptr;
+// Work sums over one class pair's task regions, for the algorithmic FLOP model (kernel
+// orientation: the bra is the (la lb) side): acc[0] += K_ab K_cd (primitive quartets),
+// acc[1] += K_ab, acc[2] += sequential Cartesian->spherical transform cost, acc[3] += output
+// integrals in the shells' basis.  One wavefront per segment.
+__global__ void work_stats_kernel(const uint2* __restrict__ tasks, const Segment* __restrict__ segs,
+                                  const unsigned* __restrict__ nseg_ptr, const GroupRec* __restrict__ groups,
+                                  const int* __restrict__ uorder, int flip, const PairInfo* __restrict__ pairs,
+                                  const ShellMeta* __restrict__ sh, double* __restrict__ acc) {
+  const long nseg = (long)*nseg_ptr;
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const long nwaves = (long)gridDim.x * blockDim.x / 64;
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
-    const uint2 tk = tasks[t];
-    const PairInfo B = pairs[pair_of(tk.x)], K = pairs[pair_of(tk.y)];
-    int bp0, bp1, kp0, kp1;
-    prim_range(B, block_of(tk.x), bp0, bp1);
-    prim_range(K, block_of(tk.y), kp0, kp1);
-    const double kab = bp1 - bp0, kcd = kp1 - kp0;
-    s0 += kab * kcd;
-    s1 += kab;
-    // per-quartet terms once per quartet: entries of a split pair other than its first block
-    // are further primitive blocks of the same quartet
-    if (block_of(tk.x) > 1 || block_of(tk.y) > 1) continue;
-    const int shl[4] = {B.a, B.b, K.a, K.b};
-    double dims[4];
-    int l4[4], p4[4];
-    for (int k = 0; k < 4; ++k) {
-      const ShellMeta m = sh[shl[k]];
-      l4[k] = m.l; p4[k] = m.pure;
-      dims[k] = (m.l + 1) * (m.l + 2) / 2;
+  for (long si = wave; si < nseg; si += nwaves) {
+    const Segment sg = segs[si];
+    const GroupRec gr = groups[sg.ket];
+    const uint2* reg = tasks + sg.off;
+    for (int m = 0; m < gr.n; ++m) {
+      const PairInfo Up = pairs[uorder[gr.first + m]];
+      int up0, up1;
+      prim_range<UNI_BLOCK>(Up, (unsigned)sg.blk, up0, up1);
+      const uint2 hm = reg[1 + m];
+      for (int t = lane; t < (int)hm.x; t += 64) {
+        const uint2 e = reg[hm.y + t];
+        const PairInfo Sp = pairs[pair_of(e.x)];
+        int sp0, sp1;
+        prim_range(Sp, block_of(e.x), sp0, sp1);
+        const PairInfo& B = flip ? Up : Sp;
+        const PairInfo& K = flip ? Sp : Up;
+        const double kab = flip ? up1 - up0 : sp1 - sp0, kcd = flip ? sp1 - sp0 : up1 - up0;
+        s0 += kab * kcd;
+        s1 += kab;
+        // per-quartet terms once per quartet: entries of a split pair other than its first
+        // block are further primitive blocks of the same quartet
+        if (block_of(e.x) > 1 || sg.blk > 1) continue;
+        const int shl[4] = {B.a, B.b, K.a, K.b};
+        double dims[4];
+        int l4[4], p4[4];
+        for (int k = 0; k < 4; ++k) {
+          const ShellMeta sm = sh[shl[k]];
+          l4[k] = sm.l; p4[k] = sm.pure;
+          dims[k] = (sm.l + 1) * (sm.l + 2) / 2;
+        }
+        double cost = 0.0;
+        for (int k = 0; k < 4; ++k) {
+          if (!p4[k] || l4[k] < 2) continue;
+          double other = 1.0;
+          for (int q = 0; q < 4; ++q) if (q != k) other *= dims[q];
+          cost += 2.0 * (2 * l4[k] + 1) * dims[k] * other;
+          dims[k] = 2 * l4[k] + 1;
+        }
+        s2 += cost;
+        double nsph = 1.0;
+        for (int k = 0; k < 4; ++k) nsph *= p4[k] ? (2 * l4[k] + 1) : (l4[k] + 1) * (l4[k] + 2) / 2;
+        s3 += nsph;
+      }
     }
-    double cost = 0.0;
-    for (int k = 0; k < 4; ++k) {
-      if (!p4[k] || l4[k] < 2) continue;
-      double other = 1.0;
-      for (int q = 0; q < 4; ++q) if (q != k) other *= dims[q];
-      cost += 2.0 * (2 * l4[k] + 1) * dims[k] * other;
-      dims[k] = 2 * l4[k] + 1;
-    }
-    s2 += cost;
-    double nsph = 1.0;
-    for (int k = 0; k < 4; ++k) nsph *= p4[k] ? (2 * l4[k] + 1) : (l4[k] + 1) * (l4[k] + 2) / 2;
-    s3 += nsph;
   }
-  atomicAdd(&acc[0], s0);
-  atomicAdd(&acc[1], s1);
-  atomicAdd(&acc[2], s2);
-  atomicAdd(&acc[3], s3);
+  s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2); s3 = wave_sum(s3);
+  if (lane == 0 && s0 + s1 + s2 + s3 > 0.0) {
+    atomicAdd(&acc[0], s0);
+    atomicAdd(&acc[1], s1);
+    atomicAdd(&acc[2], s2);
+    atomicAdd(&acc[3], s3);
+  }
 }
 
 // =======================================================================================
@@ -582,13 +718,26 @@ struct cq_ctx {
   double prim_threshold = 1e-17;
   // class-pair tables: [0] every class pair ket-keyed, [1] with bra-keyed (flipped) class pairs
   std::vector<CPMeta> cps[2];
-  std::vector<int> item_start[2];
-  int nitem[2] = {0, 0};
   long long task_capacity = 0;
   // task lists per class pair are at most task_budget entries (8 B each); a class pair whose
-  // worst case is larger screens and digests its uniform pairs in batches:
-  // batch_start[t][k] holds the class-local uniform-pair boundaries (nbatch + 1 of them)
+  // worst case is larger screens and digests its groups in batches:
+  // batch_start[t][k] holds the class-pair-local group boundaries (nbatch + 1 of them)
   long long task_budget = 1LL << 28;
+  // uniform groups: uorder lists every class's pairs (global indices, at class_start) in the
+  // order groups are cut from (molecule, primitive blocks, first shell, second shell);
+  // groups_h[t] the groups of every class pair of table t (CPMeta::grp_off)
+  std::vector<int> uorder_h;
+  std::vector<GroupRec> groups_h[2];
+  DevBuf<int> d_uorder;
+  DevBuf<GroupRec> d_groups[2];
+  // per class pair: dynamic LDS budget of the group kernels (bytes) and an optional cap on
+  // the group size and on the union bound (0: from the budget)
+  std::vector<long long> tile_lds = std::vector<long long>(CQ_NUM_CLASS_PAIRS, 0);
+  std::vector<int> tile_mmax = std::vector<int>(CQ_NUM_CLASS_PAIRS, 0);
+  std::vector<int> tile_ub = std::vector<int>(CQ_NUM_CLASS_PAIRS, 0);
+  // per class pair: the digestion table of the last contraction (work statistics)
+  std::vector<int> last_table = std::vector<int>(CQ_NUM_CLASS_PAIRS, 0);
+  bool plan_ok = false;   // task and segment buffers are allocated for the current plan
   std::vector<std::vector<int>> batch_start[2];
   std::vector<unsigned char> nblk_h, nublk_h;
   std::vector<int> pair_mol_h;
@@ -596,9 +745,9 @@ struct cq_ctx {
   // profiling: events around every class-pair kernel of every contraction
   bool profiling = false;
   std::vector<int> launch_order;
-  std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof_events;
-  std::vector<double> prof_ms;
-  std::vector<long long> prof_launches;
+  std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof_events, prof_screen_events;
+  std::vector<double> prof_ms, prof_screen_ms;
+  std::vector<long long> prof_launches, prof_screen_launches;
   std::vector<hipEvent_t> event_pool;
   size_t event_next = 0;
   hipEvent_t get_event() {
@@ -618,8 +767,8 @@ struct cq_ctx {
   DevBuf<double> d_alpha, d_coef, d_origin, d_onee;
   // device
   DevBuf<ShellMeta> d_shells;
-  DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_item_start[2], d_flags;
-  DevBuf<double> d_tco, d_boys, d_pairQ, d_Nmax, d_work, d_mats;
+  DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_flags;
+  DevBuf<double> d_tco, d_boys, d_pairQ, d_Nmax, d_Npair, d_work, d_mats;
   DevBuf<PairInfo> d_pairs, d_pairs_1e;
   DevBuf<int2> d_s12;
   std::vector<int> grp, cls_grp_off;
@@ -650,8 +799,8 @@ struct cq_ctx {
 
   ~cq_ctx() {
     d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
-    d_item_start[0].release(); d_item_start[1].release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
-    d_Nmax.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_pairs_1e.release(); d_prims.release();
+    d_uorder.release(); d_groups[0].release(); d_groups[1].release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
+    d_Nmax.release(); d_Npair.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_pairs_1e.release(); d_prims.release();
     d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_qcounts.release(); d_nglob.release(); d_pair_nblk.release(); d_pair_nublk.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
@@ -682,32 +831,75 @@ int persistent_grid() {
   return g;
 }
 
-// Class-pair metadata, task-list capacities and batches of uniform pairs.  Table 1 keys the
-// segments of a class pair on the bra pair when the bra class has at most half as many pairs
-// as the ket class (few bras survive per ket, many kets per bra) and the bra-keyed LDS
-// accumulators fit.  A uniform pair's worst case is every swept pair of its own molecule
-// (same class: from the uniform pair on), one task entry per primitive block of a swept pair.
-// Consecutive uniform pairs form one batch while their worst cases sum to at most
-// task_budget, so a class pair's task list never exceeds max(task_budget, one uniform pair's
-// worst case) whatever the density; a batch is screened and digested before the next one.
+// Dynamic LDS bytes of a group kernel: ket-keyed with LDS K columns (variant 1), with HBM K
+// atomics (variant 0), or bra-keyed (variant 2), for groups of up to mmax members and unions of
+// up to ub swept pairs.
+size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int ub) {
+  const size_t ncd = (size_t)ck.nc * ck.nd, nab = (size_t)ck.na * ck.nb;
+  // (the last term: one int per union slot, its swept pair)
+  if (variant == 2)
+    return (((size_t)ck.na + (size_t)mmax * ck.nb) * lds_n + mmax * nab + ub * ncd) * sizeof(double) + ub * sizeof(int);
+  const size_t cols = variant == 1 ? ((size_t)ck.nc + (size_t)mmax * ck.nd) * lds_n : 0;
+  return (cols + mmax * ncd + ub * nab) * sizeof(double) + ub * sizeof(int);
+}
+
+// Tile of a class pair for one digestion variant: the union bound (J slots of the swept side,
+// 64 .. 1024 pairs) and the largest group (at most the cap, default 1) that keeps the kernel's
+// dynamic LDS within the budget (default 48 KiB: with the 26 KiB Boys table two workgroups share
+// a CU).  mmax = 0: the variant does not fit the device even with single-pair groups.
+// Default union bound per class pair (kernel orientation), from a sweep of 256..1024 with
+// single-pair groups on (H2O)16/cc-pVDZ, tau = 1e-12 (per-class kernel + screening time): the
+// bound trades fewer K-column flushes (large) against more work items to balance (small).
+int default_union_bound(const ClassKernels& ck, size_t jside) {
+  static const struct { const char* name; int ub; } tab[] = {
+      {"ssss", 512}, {"ssps", 1024}, {"psps", 1024}, {"sspp", 1024}, {"ppps", 512}, {"pppp", 256},
+      {"ssds", 1024}, {"psds", 1024}, {"ppds", 256}, {"dsds", 256}, {"ssdp", 512}, {"dpps", 512},
+      {"dppp", 1024}, {"dpds", 256}, {"dpdp", 512}, {"ddss", 256}, {"ddps", 512}, {"ddpp", 512},
+      {"ddds", 512}, {"dddp", 512}, {"dddd", 256}};
+  for (const auto& t : tab)
+    if (std::strcmp(t.name, ck.name) == 0) return t.ub;
+  return (int)std::min<size_t>(UNION_MAX, std::max<size_t>(64, (16 * 1024) / (jside * sizeof(double)) / 64 * 64));
+}
+
+void class_tile(const cq_ctx* c, int k, int variant, int& mmax, int& ub) {
+  const ClassKernels& ck = kClassKernels[k];
+  const size_t jside = variant == 2 ? (size_t)ck.nc * ck.nd : (size_t)ck.na * ck.nb;
+  ub = default_union_bound(ck, jside);
+  if (c->tile_ub[k] > 0) ub = std::max(64, std::min(c->tile_ub[k], UNION_MAX));
+  const size_t budget = c->tile_lds[k] > 0 ? (size_t)c->tile_lds[k] : (size_t)48 * 1024;
+  const size_t limit = std::min(budget, c->max_dyn_lds);
+  // a union bound whose J slots alone overflow the device's LDS is lowered (multiples of 64)
+  while (ub > 64 && group_lds(ck, variant, c->lds_n, 1, ub) > c->max_dyn_lds) ub -= 64;
+  mmax = 0;
+  // single-pair groups by default: on (H2O)16/cc-pVDZ larger groups cut the J updates but cost
+  // as much again in screening (every member is tested against the group's whole sweep)
+  const int cap = c->tile_mmax[k] > 0 ? std::min(c->tile_mmax[k], MAX_GROUP) : 1;
+  for (int mm = cap; mm >= 1; --mm)
+    if (group_lds(ck, variant, c->lds_n, mm, ub) <= limit) { mmax = mm; break; }
+  if (mmax == 0 && group_lds(ck, variant, c->lds_n, 1, ub) <= c->max_dyn_lds) mmax = 1;
+}
+
+// Class-pair metadata, uniform groups, task-list capacities and batches of groups.  Table 1 keys
+// the groups of a class pair on the bra pairs when the bra class has at most half as many pairs
+// as the ket class (few bras survive per ket, many kets per bra) and the bra-keyed LDS rows fit.
+// A group's worst case is every primitive block of every swept pair of its molecule (same
+// class: from its first member on) in every member's list, and a header per flushed task
+// region.  Consecutive groups form one batch while their worst cases sum to at most
+// task_budget, so a class pair's task list never exceeds max(task_budget, one group's worst
+// case) whatever the density; a batch is screened and digested before the next one.
 void plan_tasks(cq_ctx* c) {
+  c->plan_ok = false;
   const int nm = c->nmol;
   const std::vector<unsigned char>& nblk = c->nblk_h;
   const std::vector<unsigned char>& nublk = c->nublk_h;
   const std::vector<int>& pair_mol = c->pair_mol_h;
   for (int t = 0; t < 2; ++t) {
     c->cps[t].clear();
-    c->item_start[t].clear();
+    c->groups_h[t].clear();
     c->batch_start[t].assign(kNumClassPairs, std::vector<int>());
   }
-  auto class_max = [&](int cls, const std::vector<unsigned char>& nb) {
-    int mx = 1;
-    for (int k = c->class_start[cls]; k < c->class_start[cls + 1]; ++k) mx = std::max<int>(mx, nb[k]);
-    return mx;
-  };
   const long long budget = c->task_budget;
   long long off = 0, seg_total[2] = {0, 0};
-  int items[2] = {0, 0};
   for (int k = 0; k < kNumClassPairs; ++k) {
     const ClassKernels& ck = kClassKernels[k];
     CPMeta m{};
@@ -718,60 +910,95 @@ void plan_tasks(cq_ctx* c) {
     m.same = ck.i == ck.j;
     m.bra_class = ck.bi;
     m.ket_class = ck.ki;
-    // swept-side primitive blocks per molecule
-    std::vector<long long> bsum_m(nm, 0), ksum_m(nm, 0);
-    for (int i = 0; i < m.nbra; ++i) bsum_m[pair_mol[m.bra_start + i]] += nblk[m.bra_start + i];
-    for (int i = 0; i < m.nket; ++i) ksum_m[pair_mol[m.ket_start + i]] += nblk[m.ket_start + i];
-    const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
-    const int flip = (!m.same && ck.jksegb && ldsb <= c->max_dyn_lds && 2LL * m.nbra <= m.nket) ? 1 : 0;
+    // swept-side pairs and primitive blocks per molecule
+    std::vector<long long> bp_m(nm, 0), kp_m(nm, 0), bb_m(nm, 0), kb_m(nm, 0);
+    for (int i = 0; i < m.nbra; ++i) { bp_m[pair_mol[m.bra_start + i]] += 1; bb_m[pair_mol[m.bra_start + i]] += nblk[m.bra_start + i]; }
+    for (int i = 0; i < m.nket; ++i) { kp_m[pair_mol[m.ket_start + i]] += 1; kb_m[pair_mol[m.ket_start + i]] += nblk[m.ket_start + i]; }
+    int mm2 = 0, ub2 = 64;
+    if (ck.grp[0][2]) class_tile(c, k, 2, mm2, ub2);
+    const int flip = (!m.same && ck.grp[0][2] && mm2 > 0 && 2LL * m.nbra <= m.nket) ? 1 : 0;
     CPMeta mt[2] = {m, m};
     long long region = 0;
     for (int t = 0; t < 2; ++t) {
-      mt[t].flip = t == 1 ? flip : 0;
-      const int uni_start = mt[t].flip ? m.bra_start : m.ket_start;
-      const int nuni = mt[t].flip ? m.nbra : m.nket;
-      std::vector<long long> w(nuni);
-      if (m.same) {
-        // swept pairs i >= u of u's molecule (a molecule's pairs are contiguous in a class)
-        long long run = 0;
-        for (int u = nuni - 1; u >= 0; --u) {
-          if (u + 1 < nuni && pair_mol[uni_start + u] != pair_mol[uni_start + u + 1]) run = 0;
-          run += nblk[uni_start + u];
-          w[u] = run;
-        }
+      CPMeta& mc = mt[t];
+      mc.flip = t == 1 ? flip : 0;
+      int mmax = 0, ub = 64;
+      if (mc.flip) {
+        mmax = mm2;
+        ub = ub2;
       } else {
-        const std::vector<long long>& sw = mt[t].flip ? ksum_m : bsum_m;
-        for (int u = 0; u < nuni; ++u) w[u] = sw[pair_mol[uni_start + u]];
+        class_tile(c, k, 1, mmax, ub);
+        if (mmax == 0) class_tile(c, k, 0, mmax, ub);   // K columns do not fit: HBM K atomics
+        if (mmax == 0) throw std::invalid_argument(std::string("class pair ") + ck.name + ": no tile fits the device's LDS");
       }
-      // segments: a uniform pair with T tasks is flushed at most T / (SCREEN_BUF - 64) + 1 times
-      // (a flush happens once fewer than 64 slots are left), each flush writing one segment per
-      // primitive block of the uniform pair
-      const int umax = class_max(mt[t].flip ? m.bra_class : m.ket_class, nublk);
+      mc.mmax = mmax;
+      mc.ub = ub;
+      const int ucls = mc.flip ? m.bra_class : m.ket_class;
+      const int uni_start = mc.flip ? m.bra_start : m.ket_start;
+      // groups: runs of the class's uniform order with one molecule, one primitive-block count
+      // and one first shell, at most mmax long
+      mc.grp_off = (int)c->groups_h[t].size();
+      const int u0 = c->class_start[ucls], u1 = c->class_start[ucls + 1];
+      for (int x = u0; x < u1;) {
+        const int P0 = c->uorder_h[x];
+        int y = x + 1;
+        while (y < u1 && y - x < mmax) {
+          const int P = c->uorder_h[y];
+          if (pair_mol[P] != pair_mol[P0] || nublk[P] != nublk[P0] || c->pairs[P].a != c->pairs[P0].a) break;
+          ++y;
+        }
+        c->groups_h[t].push_back(GroupRec{x, y - x, nublk[P0], 0});
+        x = y;
+      }
+      mc.ngrp = (int)c->groups_h[t].size() - mc.grp_off;
+      // suffix counts of the swept class (same class: from the group's first member on)
+      std::vector<long long> suf_p, suf_b;
+      if (m.same) {
+        suf_p.assign(m.nbra + 1, 0);
+        suf_b.assign(m.nbra + 1, 0);
+        for (int i = m.nbra - 1; i >= 0; --i) {
+          const bool brk = i + 1 < m.nbra && pair_mol[m.bra_start + i] != pair_mol[m.bra_start + i + 1];
+          suf_p[i] = 1 + (brk ? 0 : suf_p[i + 1]);
+          suf_b[i] = nblk[m.bra_start + i] + (brk ? 0 : suf_b[i + 1]);
+        }
+      }
       std::vector<int>& bs = c->batch_start[t][k];
       bs.assign(1, 0);
       long long cap = 0, cur = 0, seg_cap = 0, seg_cur = 0;
-      auto close = [&](int u) {
-        bs.push_back(u);
+      auto close = [&](int gi) {
+        bs.push_back(gi);
         cap = std::max(cap, cur);
-        seg_cap = std::max(seg_cap, seg_cur + (cur / (SCREEN_BUF - 64) + 1) * umax);
+        seg_cap = std::max(seg_cap, seg_cur);
         cur = 0;
         seg_cur = 0;
       };
-      for (int u = 0; u < nuni; ++u) {
-        if (u > bs.back() && cur + w[u] > budget) close(u);
-        cur += w[u];
-        seg_cur += nublk[uni_start + u];
+      for (int gi = 0; gi < mc.ngrp; ++gi) {
+        const GroupRec& gr = c->groups_h[t][mc.grp_off + gi];
+        const int mol = pair_mol[c->uorder_h[gr.first]];
+        long long S, Bk;
+        if (m.same) {
+          int lmin = 0x7fffffff;
+          for (int q = 0; q < gr.n; ++q) lmin = std::min(lmin, c->uorder_h[gr.first + q] - uni_start);
+          S = suf_p[lmin];
+          Bk = suf_b[lmin];
+        } else {
+          S = mc.flip ? kp_m[mol] : bp_m[mol];
+          Bk = mc.flip ? kb_m[mol] : bb_m[mol];
+        }
+        const long long nflush = S / (ub - 63) + 1;
+        const long long w = (long long)gr.n * Bk + nflush * (1 + gr.n);
+        if (gi > bs.back() && cur + w > budget) close(gi);
+        cur += w;
+        seg_cur += nflush * gr.ublk;
       }
-      close(nuni);
+      close(mc.ngrp);
       // task offsets within a list are 32-bit (Segment::off)
-      if (cap > 0x7fffffffLL) throw std::invalid_argument("one shell pair's task list exceeds 2^31 entries");
-      mt[t].task_cap = cap;
-      mt[t].seg_cap = seg_cap;
-      mt[t].seg_off = seg_total[t];
+      if (cap > 0x7fffffffLL) throw std::invalid_argument("one group's task list exceeds 2^31 entries");
+      mc.task_cap = cap;
+      mc.seg_cap = seg_cap;
+      mc.seg_off = seg_total[t];
       seg_total[t] += seg_cap;
       region = std::max(region, cap);
-      c->item_start[t].push_back(items[t]);
-      items[t] += nuni;
     }
     for (int t = 0; t < 2; ++t) {
       mt[t].task_off = off;
@@ -784,13 +1011,13 @@ void plan_tasks(cq_ctx* c) {
   // variants): table 1's segments live after table 0's
   for (auto& mt : c->cps[1]) mt.seg_off += seg_total[0];
   for (int t = 0; t < 2; ++t) {
-    c->nitem[t] = items[t];
     c->d_cps[t].upload(c->cps[t], c->stream);
-    c->d_item_start[t].upload(c->item_start[t], c->stream);
+    c->d_groups[t].upload(c->groups_h[t], c->stream);
   }
   c->d_tasks.ensure((size_t)std::max<long long>(1, off));
   c->d_segs.ensure((size_t)std::max<long long>(1, seg_total[0] + seg_total[1]));
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->plan_ok = true;
 }
 
 void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
@@ -1081,15 +1308,32 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_pair_nublk.upload(nublk, c->stream);
   c->nblk_h.swap(nblk);
   c->nublk_h.swap(nublk);
+  // uniform order of every class: by molecule, primitive blocks, first and second shell (groups
+  // are cut from runs sharing the first three)
+  c->uorder_h.resize(npair);
+  std::iota(c->uorder_h.begin(), c->uorder_h.end(), 0);
+  for (int X = 0; X < c->npc; ++X)
+    std::stable_sort(c->uorder_h.begin() + c->class_start[X], c->uorder_h.begin() + c->class_start[X + 1],
+                     [&](int x, int y) {
+                       if (c->pair_mol_h[x] != c->pair_mol_h[y]) return c->pair_mol_h[x] < c->pair_mol_h[y];
+                       if (c->nublk_h[x] != c->nublk_h[y]) return c->nublk_h[x] < c->nublk_h[y];
+                       if (c->pairs[x].a != c->pairs[y].a) return c->pairs[x].a < c->pairs[y].a;
+                       return c->pairs[x].b < c->pairs[y].b;
+                     });
+  c->d_uorder.upload(c->uorder_h, c->stream);
   plan_tasks(c);
   c->d_nseg.ensure(kNumClassPairs);
-  c->d_work_q.ensure(kNumClassPairs);
+  c->d_work_q.ensure((size_t)kNumClassPairs * MAX_MATS);   // one work queue per class pair and slot pass
   c->d_counts.ensure(kNumClassPairs);
   c->d_qcounts.ensure(kNumClassPairs);
-  c->d_nglob.ensure(1);
+  c->d_nglob.ensure(2 + MAX_MATS);   // largest |X| of the call, of each matrix, norm asymmetry flag
   c->d_Nmax.ensure((size_t)nshell * nshell);
+  c->d_Npair.ensure(std::max(1, npair));
   c->last_counts.assign(kNumClassPairs, 0);
   c->prof_events.assign(kNumClassPairs, {});
+  c->prof_screen_events.assign(kNumClassPairs, {});
+  c->prof_screen_ms.assign(kNumClassPairs, 0.0);
+  c->prof_screen_launches.assign(kNumClassPairs, 0);
   c->launch_order.resize(kNumClassPairs);
   std::iota(c->launch_order.begin(), c->launch_order.end(), 0);
   std::stable_sort(c->launch_order.begin(), c->launch_order.end(),
@@ -1108,6 +1352,7 @@ struct Slot {
 void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const* AX, const int* types,
                      const double* scales, int n, int ld, hipStream_t s) {
   if (nmat < 1 || nmat > MAX_MATS) throw std::invalid_argument("nmat must be in 1..8");
+  if (!c->plan_ok) throw std::runtime_error("no task buffers: the last cq_set_task_budget could not allocate a plan");
   if (n != c->nsph) throw std::invalid_argument("matrix dimension does not match the basis");
   if (ld < n) throw std::invalid_argument("leading dimension smaller than the matrix dimension");
   for (int i = 0; i < nmat; ++i) {
@@ -1133,9 +1378,13 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   const int ns = c->nshell;
   const long nc2 = (long)c->ncart * c->ncart;
   const int nslot = (int)slots.size();
-  // workspace: per slot Dc, GJ, GJ2, GK, GK2
-  c->d_work.ensure((size_t)std::max(1, nslot) * 5 * nc2);
+  // workspace: per slot Dc, GJ, GJ2, GK, GK2, then one flag per slot
+  const size_t work_words = (size_t)std::max(1, nslot) * 5 * nc2;
+  c->d_work.ensure(work_words);
   c->d_flags.ensure(std::max(1, nslot));
+  double* work = c->d_work.p;
+  int* flags = c->d_flags.p;
+  const bool profiling = c->profiling;
   // 1. one launch zeroes the accumulators, counters and flags
   ZeroList zl{};
   auto zero = [&](void* p, size_t bytes) {
@@ -1143,14 +1392,14 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
     zl.n[zl.count] = (long)(bytes / 4);
     ++zl.count;
   };
-  zero(c->d_nglob.p, sizeof(unsigned long long));
-  zero(c->d_flags.p, std::max(1, nslot) * sizeof(int));
+  zero(c->d_nglob.p, (2 + MAX_MATS) * sizeof(unsigned long long));
+  zero(flags, std::max(1, nslot) * sizeof(int));
   zero(c->d_counts.p, kNumClassPairs * sizeof(unsigned long long));
   zero(c->d_qcounts.p, kNumClassPairs * sizeof(unsigned long long));
   zero(c->d_nseg.p, kNumClassPairs * sizeof(unsigned));
-  zero(c->d_work_q.p, kNumClassPairs * sizeof(unsigned));
+  zero(c->d_work_q.p, (size_t)kNumClassPairs * MAX_MATS * sizeof(unsigned));
   // the whole workspace (the Cartesian densities in it are rewritten below)
-  zero(c->d_work.p, (size_t)std::max(1, nslot) * 5 * nc2 * sizeof(double));
+  zero(work, (size_t)std::max(1, nslot) * 5 * nc2 * sizeof(double));
   hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(WG), 0, s, zl);
   HIPCHK(hipGetLastError());
   // 2. block norms over every input matrix (scale 0 included)
@@ -1160,25 +1409,30 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   hipLaunchKernelGGL(norm_kernel, dim3(grid_for((long)ns * ns, WG)), dim3(WG), 0, s, ml, ld, c->d_shells.p, ns,
                      c->d_Nmax.p, c->d_nglob.p);
   HIPCHK(hipGetLastError());
+  {
+    const int np = (int)c->pairs.size();
+    hipLaunchKernelGGL(pair_norm_kernel, dim3(grid_for(np, WG)), dim3(WG), 0, s, c->d_s12.p, np, c->d_Nmax.p, ns,
+                       c->d_Npair.p);
+    HIPCHK(hipGetLastError());
+  }
   if (nslot == 0) return;
   // 3. Cartesian densities and symmetry flags; per slot: [Dc | GJ | GJ2 | GK | GK2], ncart^2 each
   for (int k = 0; k < nslot; ++k) {
-    double* base = c->d_work.p + (size_t)k * 5 * nc2;
+    double* base = work + (size_t)k * 5 * nc2;
     hipLaunchKernelGGL(cart_density_kernel, dim3(grid_for(std::max<long>(nc2, (long)n * n), WG)), dim3(WG), 0, s,
                        slots[k].X, n, ld, c->d_shells.p, c->d_cart2shell.p, c->ncart, c->d_tco.p, c->d_tco_off.p,
-                       c->d_nglob.p, base, c->d_flags.p + k);
+                       c->d_nglob.p + 1 + (std::find(uniq.begin(), uniq.end(), slots[k].X) - uniq.begin()), base,
+                       flags + k);
     HIPCHK(hipGetLastError());
   }
-  // digestion variant per class pair (the lds_mode, or the class's own choice below it);
-  // bra-keyed segments (task table 1) need one density slot
+  // digestion variant per class pair (the lds_mode, or the class's own choice below it)
   auto variant_of = [&](int k) {
     const int v = c->class_variant[k] < 0 ? c->lds_mode : std::min(c->class_variant[k], c->lds_mode);
     return v;
   };
   // mixed precision: a second round screens the bound window [tau, threshold) into the same
-  // task buffers and runs it with FP32 integrals (one density slot on the segment path only)
-  const bool mixed = c->precision_mode == CQ_PREC_MIXED && nslot == 1 && c->lds_mode >= 1 &&
-                     c->mixed_threshold > c->tau;
+  // task buffers and runs it with FP32 integrals (one density slot)
+  const bool mixed = c->precision_mode == CQ_PREC_MIXED && nslot == 1 && c->mixed_threshold > c->tau;
   const int nround = mixed ? 2 : 1;
   for (int round = 0; round < nround; ++round) {
     const bool fp32 = round == 1;
@@ -1194,29 +1448,27 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       };
       zero2(c->d_counts.p, kNumClassPairs * sizeof(unsigned long long));
       zero2(c->d_nseg.p, kNumClassPairs * sizeof(unsigned));
-      zero2(c->d_work_q.p, kNumClassPairs * sizeof(unsigned));
+      zero2(c->d_work_q.p, (size_t)kNumClassPairs * MAX_MATS * sizeof(unsigned));
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(WG), 0, s, zr);
       HIPCHK(hipGetLastError());
     }
   // 4. per class pair (on the auxiliary streams, highest angular momentum first: those are the
-  //    latency-bound ones): screening into its compacted task list, then ERI + J/K digestion,
-  //    two density slots per pass; joined back into s before the finalisation
-  for (int s0 = 0; s0 < nslot; s0 += MAX_SLOTS) {
-    const int ng = std::min(MAX_SLOTS, nslot - s0);
+  //    latency-bound ones): screening into its task regions, then ERI + J/K digestion, one
+  //    density slot per pass (the lists are screened once and kept for every slot when the class
+  //    pair is one batch); joined back into s before the finalisation
+  for (int s0 = 0; s0 < nslot; ++s0) {
     JKArgs jk{};
     jk.ncart = c->ncart;
     jk.lds_n = c->lds_n;
-    for (int g = 0; g < ng; ++g) {
-      double* base = c->d_work.p + (size_t)(s0 + g) * 5 * nc2;
-      jk.D[g] = base;
-      jk.GJ[g] = base + nc2;
-      jk.GJ2[g] = base + 2 * nc2;
-      jk.GK[g] = base + 3 * nc2;
-      jk.GK2[g] = base + 4 * nc2;
-      jk.do_j[g] = slots[s0 + g].do_j;
-      jk.do_k[g] = slots[s0 + g].do_k;
-    }
-    jk.nonsym = c->d_flags.p + s0;
+    double* base = work + (size_t)s0 * 5 * nc2;
+    jk.D[0] = base;
+    jk.GJ[0] = base + nc2;
+    jk.GJ2[0] = base + 2 * nc2;
+    jk.GK[0] = base + 3 * nc2;
+    jk.GK2[0] = base + 4 * nc2;
+    jk.do_j[0] = slots[s0].do_j;
+    jk.do_k[0] = slots[s0].do_k;
+    jk.nonsym = flags + s0;
     HIPCHK(hipEventRecord(c->ev_fork, s));
     for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
     int launched = 0;
@@ -1224,73 +1476,71 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const int k = c->launch_order[kk];
       const ClassKernels& ck = kClassKernels[k];
       const int var = variant_of(k);
-      const int tb = (var >= 2 && nslot == 1) ? 1 : 0;
+      const int tb = var >= 2 ? 1 : 0;
       const CPMeta& m = c->cps[tb][k];
-      if (m.nbra == 0 || m.nket == 0) continue;
+      c->last_table[k] = tb;
+      if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
       hipStream_t sk = c->class_stream[k] >= 0 ? c->aux[c->class_stream[k] % c->n_streams]
                                                : c->aux[launched++ % c->n_streams];
-      const uint2* tasks = c->d_tasks.p + m.task_off;
-      const long long cap = m.task_cap;
-      const long long items = ((cap + 63) / 64) * 64 * ck.nchunk;
-      const int grid = (int)std::min<long long>(persistent_grid(), (items + WG - 1) / WG);
-      JKKernel kern = ng == 1 ? ck.jk1 : ck.jk2;
-      // one density slot and LDS room for the K columns: segment kernel (LDS accumulation)
-      const size_t lds_bytes = ((size_t)(ck.nc + ck.nd) * c->lds_n + (size_t)ck.nc * ck.nd) * sizeof(double);
-      const bool seg_path = var >= 1 && ng == 1 && lds_bytes <= c->max_dyn_lds;
-      // batches of uniform pairs, in order on this class pair's stream: a class pair in one
-      // batch screens once and keeps its lists for every slot group; a batched one clears its
-      // counters and screens again before each batch
+      // kernel: bra-keyed groups, ket-keyed with LDS K columns where they fit, else HBM K atomics
+      const int kv = m.flip ? 2 : (var >= 1 && group_lds(ck, 1, c->lds_n, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
+      const size_t lds_bytes = group_lds(ck, kv, c->lds_n, m.mmax, m.ub);
+      if (m.mmax < 1 || lds_bytes > c->max_dyn_lds)
+        throw std::runtime_error(std::string("class pair ") + ck.name + ": the planned tile does not fit the device's LDS");
+      GroupKernel kern = ck.grp[fp32 ? 1 : 0][kv];
+      const int wgs = kv == 2 ? WG : ck.seg_wg;
+      unsigned* wq = c->d_work_q.p + (size_t)s0 * kNumClassPairs + k;
+      // batches of groups, in order on this class pair's stream: a class pair in one batch
+      // screens once and keeps its lists for every slot pass; a batched one clears its counters
+      // and screens again before each batch
       const std::vector<int>& bs = c->batch_start[tb][k];
       const int nbatch = (int)bs.size() - 1;
       for (int b = 0; b < nbatch; ++b) {
-        const int u0 = bs[b], nu = bs[b + 1] - bs[b];
+        const int g0 = bs[b], ng = bs[b + 1] - bs[b];
         if (nbatch > 1) {
           ZeroList zb{};
           zb.p[0] = (unsigned*)(c->d_counts.p + k);
           zb.n[0] = 2;
           zb.p[1] = c->d_nseg.p + k;
           zb.n[1] = 1;
-          zb.p[2] = c->d_work_q.p + k;
+          zb.p[2] = wq;
           zb.n[2] = 1;
           zb.count = 3;
           hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, sk, zb);
           HIPCHK(hipGetLastError());
         }
         if (s0 == 0 || nbatch > 1) {
-          hipLaunchKernelGGL(screen_kernel, dim3(grid_for(nu, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p,
-                             kNumClassPairs, c->d_item_start[tb].p, nu, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
-                             c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p, c->nmol, c->d_Nmax.p, c->d_nglob.p, ns,
-                             lo, hi, c->d_pair_nblk.p, c->d_pair_nublk.p, c->d_tasks.p, c->d_counts.p,
-                             c->d_qcounts.p, c->d_segs.p, c->d_nseg.p, k, u0, s0 == 0 ? 1 : 0);
+          hipEvent_t q0 = nullptr, q1 = nullptr;
+          if (profiling) {
+            q0 = c->get_event();
+            q1 = c->get_event();
+            HIPCHK(hipEventRecord(q0, sk));
+          }
+          hipLaunchKernelGGL(screen_kernel, dim3(grid_for(ng, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p, k, g0, ng,
+                             c->d_groups[tb].p, c->d_uorder.p, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
+                             c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p, c->nmol, c->d_Nmax.p, c->d_Npair.p,
+                             c->d_nglob.p, ns,
+                             lo, hi, c->d_pair_nblk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p, c->d_segs.p,
+                             c->d_nseg.p, s0 == 0 ? 1 : 0);
           HIPCHK(hipGetLastError());
+          if (profiling) {
+            HIPCHK(hipEventRecord(q1, sk));
+            c->prof_screen_events[k].push_back({q0, q1});
+          }
         }
         // profiling times the ERI + digestion kernel of each batch (not the screen)
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (c->profiling) {
+        if (profiling) {
           e0 = c->get_event();
           e1 = c->get_event();
           HIPCHK(hipEventRecord(e0, sk));
         }
-        if (m.flip) {
-          // bra-keyed segments (table 1 only flips where the LDS rows fit)
-          const size_t ldsb = ((size_t)(ck.na + ck.nb) * c->lds_n + (size_t)ck.na * ck.nb) * sizeof(double);
-          const int sgrid = (int)std::min<long long>(persistent_grid(), m.seg_cap * ck.nchunk);
-          hipLaunchKernelGGL(fp32 ? ck.jksegb32 : ck.jksegb, dim3(std::max(1, sgrid)), dim3(WG), ldsb, sk, tasks,
-                             c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
-                             c->d_prims.p, c->d_boys.p, jk);
-        } else if (seg_path) {
-          const int sgrid =
-              (int)std::min<long long>((long long)persistent_grid() * WG / ck.seg_wg, m.seg_cap * ck.nchunk);
-          hipLaunchKernelGGL(fp32 ? ck.jkseg32 : ck.jkseg, dim3(std::max(1, sgrid)), dim3(ck.seg_wg), lds_bytes, sk,
-                             tasks, c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_work_q.p + k, c->d_pairs.p,
-                             c->d_prims.p, c->d_boys.p, jk);
-        } else {
-          // (the FP32 round's lists run through the FP64 kernels where the segment path does not fit)
-          hipLaunchKernelGGL(kern, dim3(std::max(1, grid)), dim3(WG), 0, sk, tasks, c->d_counts.p + k, c->d_pairs.p,
-                             c->d_prims.p, c->d_boys.p, jk);
-        }
+        const int sgrid = (int)std::min<long long>((long long)persistent_grid() * WG / wgs, m.seg_cap * ck.nchunk);
+        hipLaunchKernelGGL(kern, dim3(std::max(1, sgrid)), dim3(wgs), lds_bytes, sk, c->d_tasks.p + m.task_off,
+                           c->d_segs.p + m.seg_off, c->d_nseg.p + k, wq, c->d_pairs.p, c->d_prims.p, c->d_boys.p, jk,
+                           c->d_groups[tb].p, c->d_uorder.p, m.mmax, m.ub);
         HIPCHK(hipGetLastError());
-        if (c->profiling) {
+        if (profiling) {
           HIPCHK(hipEventRecord(e1, sk));
           c->prof_events[k].push_back({e0, e1});
         }
@@ -1308,11 +1558,11 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   for (int i = 0; i < nmat; ++i) {
     const int k = mat_slot[i];
     if (k < 0) continue;
-    double* base = c->d_work.p + (size_t)k * 5 * nc2;
+    double* base = work + (size_t)k * 5 * nc2;
     fl.AX[fl.n] = AX[i];
     fl.G1[fl.n] = base + (types[i] == CQ_COULOMB ? 1 : 3) * nc2;
     fl.G2alt[fl.n] = base + (types[i] == CQ_COULOMB ? 2 : 4) * nc2;
-    fl.nonsym[fl.n] = c->d_flags.p + k;
+    fl.nonsym[fl.n] = flags + k;
     fl.f[fl.n] = 0.5 * scales[i];
     ++fl.n;
   }
@@ -1509,18 +1759,28 @@ int cq_contract_direct(cq_ctx_t* c, int nmat, const double* const* X, double* co
       if (!X[i] || !AX[i]) throw std::invalid_argument("null matrix pointer");
       double* px = c->d_mats.p + (size_t)(2 * i) * ld * n;
       double* pa = c->d_mats.p + (size_t)(2 * i + 1) * ld * n;
-      // identical host buffers map to one device buffer (same density, screened once)
-      int dup = -1;
-      for (int k = 0; k < i; ++k) if (X[k] == X[i]) dup = k;
-      if (dup >= 0) px = const_cast<double*>(dX[dup]);
+      // identical host buffers map to one device buffer: the same density is screened once,
+      // and an output listed twice (the reference's formPT passes AX = {PT, PT, Mz},
+      // include/singleslater/singleslater_fock.h:37-41) receives both contributions, added in
+      // list order by the finalisation, and is copied in and back once
+      int dupx = -1, dupa = -1;
+      for (int k = 0; k < i; ++k) {
+        if (X[k] == X[i]) dupx = k;
+        if (AX[k] == AX[i]) dupa = k;
+      }
+      if (dupx >= 0) px = const_cast<double*>(dX[dupx]);
       else HIPCHK(hipMemcpyAsync(px, X[i], bytes, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(pa, AX[i], bytes, hipMemcpyHostToDevice, c->stream));
+      if (dupa >= 0) pa = dAX[dupa];
+      else HIPCHK(hipMemcpyAsync(pa, AX[i], bytes, hipMemcpyHostToDevice, c->stream));
       dX[i] = px;
       dAX[i] = pa;
     }
     contract_device(c, nmat, dX.data(), dAX.data(), types, scales, n, ld, c->stream);
-    for (int i = 0; i < nmat; ++i)
-      HIPCHK(hipMemcpyAsync(AX[i], dAX[i], bytes, hipMemcpyDeviceToHost, c->stream));
+    for (int i = 0; i < nmat; ++i) {
+      bool first = true;
+      for (int k = 0; k < i; ++k) if (AX[k] == AX[i]) first = false;
+      if (first) HIPCHK(hipMemcpyAsync(AX[i], dAX[i], bytes, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
@@ -1638,14 +1898,70 @@ int cq_set_task_budget(cq_ctx_t* c, long long entries) {
     try {
       plan_tasks(c);
     } catch (...) {
-      // (out of device memory for the new plan) keep the context usable with the old one
+      // (out of device memory for the new plan) keep the context usable with the old one; if
+      // even that cannot be allocated, plan_ok stays false and contractions raise an error
+      // instead of screening into released task buffers
       c->task_budget = old;
       c->d_tasks.release();
       c->d_segs.release();
-      plan_tasks(c);
+      try {
+        plan_tasks(c);
+      } catch (...) {
+      }
       throw;
     }
   });
+}
+
+// Tile of class pair k's group kernels: the dynamic LDS budget in bytes (0: the default 48 KiB),
+// the largest group (members of one work item, 0: as many as the budget allows, at most 16) and
+// the union bound of a task region (swept pairs whose J collects in LDS slots, 0: about 16 KiB of
+// slots).  Waits for the context's work and re-plans the task lists.
+int cq_set_class_tile(cq_ctx_t* c, int k, long long lds_bytes, int max_group, int union_bound) {
+  return guarded(c, [&] {
+    if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");
+    if (lds_bytes < 0 || max_group < 0 || max_group > MAX_GROUP || union_bound < 0 || union_bound > UNION_MAX ||
+        (union_bound > 0 && union_bound < 64))
+      throw std::invalid_argument("tile out of range: lds_bytes >= 0, max_group 0..16, union_bound 0 or 64..1024");
+    for (int a = 0; a < cq_ctx::kAux; ++a)
+      if (c->aux[a]) HIPCHK(hipStreamSynchronize(c->aux[a]));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long long o0 = c->tile_lds[k];
+    const int o1 = c->tile_mmax[k], o2 = c->tile_ub[k];
+    c->tile_lds[k] = lds_bytes;
+    c->tile_mmax[k] = max_group;
+    c->tile_ub[k] = union_bound;
+    try {
+      plan_tasks(c);
+    } catch (...) {
+      c->tile_lds[k] = o0;
+      c->tile_mmax[k] = o1;
+      c->tile_ub[k] = o2;
+      try {
+        plan_tasks(c);
+      } catch (...) {
+      }
+      throw;
+    }
+  });
+}
+
+// The planned tile of class pair k in the digestion table of the given variant (0/1: ket-keyed,
+// 2: bra-keyed where the class pair has that form): largest group, union bound, number of
+// groups, and the kernel's dynamic LDS bytes.
+int cq_get_class_tile(const cq_ctx_t* c, int k, int variant, int* max_group, int* union_bound, int* ngroups,
+                      long long* lds_bytes) {
+  if (!c || k < 0 || k >= kNumClassPairs || variant < 0 || variant > 2 || !max_group || !union_bound || !ngroups ||
+      !lds_bytes)
+    return CQ_ERR_INVALID;
+  const CPMeta& m = c->cps[variant == 2 ? 1 : 0][k];
+  const ClassKernels& ck = kClassKernels[k];
+  *max_group = m.mmax;
+  *union_bound = m.ub;
+  *ngroups = m.ngrp;
+  const int kv = m.flip ? 2 : (variant >= 1 && group_lds(ck, 1, c->lds_n, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
+  *lds_bytes = (long long)group_lds(ck, kv, c->lds_n, m.mmax, m.ub);
+  return CQ_OK;
 }
 
 // Task entries allocated over all class pairs, and the largest number of batches any class
@@ -1674,30 +1990,57 @@ int cq_set_profiling(cq_ctx_t* c, int on) {
   return guarded(c, [&] {
     c->profiling = on != 0;
     for (auto& v : c->prof_events) v.clear();
+    for (auto& v : c->prof_screen_events) v.clear();
     std::fill(c->prof_ms.begin(), c->prof_ms.end(), 0.0);
     std::fill(c->prof_launches.begin(), c->prof_launches.end(), 0);
+    std::fill(c->prof_screen_ms.begin(), c->prof_screen_ms.end(), 0.0);
+    std::fill(c->prof_screen_launches.begin(), c->prof_screen_launches.end(), 0);
     c->event_next = 0;
   });
 }
 
 // Accumulated kernel milliseconds and launch counts per class pair since profiling was
 // switched on (synchronises every recorded event).
+namespace {
+// Fold the recorded event pairs of every class pair into the accumulated times (synchronises
+// every recorded event); the event pool is reused afterwards.
+void collect_times(cq_ctx* c) {
+  auto fold = [](std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs, double& ms, long long& n) {
+    for (auto& pr : evs) {
+      HIPCHK(hipEventSynchronize(pr.second));
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, pr.first, pr.second));
+      ms += t;
+      n += 1;
+    }
+    evs.clear();
+  };
+  for (int k = 0; k < kNumClassPairs; ++k) {
+    fold(c->prof_events[k], c->prof_ms[k], c->prof_launches[k]);
+    fold(c->prof_screen_events[k], c->prof_screen_ms[k], c->prof_screen_launches[k]);
+  }
+  c->event_next = 0;
+}
+}  // namespace
+
 int cq_get_kernel_times(cq_ctx_t* c, double* ms, long long* launches, int max_n) {
   return guarded(c, [&] {
-    for (int k = 0; k < kNumClassPairs; ++k) {
-      for (auto& pr : c->prof_events[k]) {
-        HIPCHK(hipEventSynchronize(pr.second));
-        float t = 0.f;
-        HIPCHK(hipEventElapsedTime(&t, pr.first, pr.second));
-        c->prof_ms[k] += t;
-        c->prof_launches[k] += 1;
-      }
-      c->prof_events[k].clear();
-    }
-    c->event_next = 0;
+    collect_times(c);
     for (int k = 0; k < kNumClassPairs && k < max_n; ++k) {
       ms[k] = c->prof_ms[k];
       launches[k] = c->prof_launches[k];
+    }
+  });
+}
+
+// Accumulated screening-kernel milliseconds and launch counts per class pair since profiling
+// was switched on (HIP events around each screen_kernel launch on its stream).
+int cq_get_screen_times(cq_ctx_t* c, double* ms, long long* launches, int max_n) {
+  return guarded(c, [&] {
+    collect_times(c);
+    for (int k = 0; k < kNumClassPairs && k < max_n; ++k) {
+      ms[k] = c->prof_screen_ms[k];
+      launches[k] = c->prof_screen_launches[k];
     }
   });
 }
@@ -1714,10 +2057,12 @@ int cq_get_work_stats(cq_ctx_t* c, double* out, int max_n) {
     acc.ensure((size_t)4 * kNumClassPairs);
     HIPCHK(hipMemsetAsync(acc.p, 0, 4 * kNumClassPairs * sizeof(double), c->stream));
     for (int k = 0; k < kNumClassPairs; ++k) {
-      if (c->cps[0][k].nbra == 0 || c->cps[0][k].nket == 0) continue;
+      const int tb = c->last_table[k];
+      const CPMeta& m = c->cps[tb][k];
+      if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
       hipLaunchKernelGGL(work_stats_kernel, dim3(persistent_grid()), dim3(WG), 0, c->stream,
-                         c->d_tasks.p + c->cps[0][k].task_off, c->d_counts.p + k, c->d_pairs.p, c->d_shells.p,
-                         acc.p + 4 * k);
+                         c->d_tasks.p + m.task_off, c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_groups[tb].p,
+                         c->d_uorder.p, m.flip, c->d_pairs.p, c->d_shells.p, acc.p + 4 * k);
       HIPCHK(hipGetLastError());
     }
     std::vector<double> h((size_t)4 * kNumClassPairs);

@@ -47,7 +47,8 @@ struct QuartetArgs {
   int loff;                  // LDS row origin: first Cartesian function of the molecule
 };
 
-#define MAX_SLOTS 2
+// One density slot per kernel launch (a call with several densities runs one pass per slot).
+#define MAX_SLOTS 1
 struct JKArgs {
   int ncart;
   int lds_n;                     // rows of an LDS accumulator column: the largest molecule's ncart

@@ -5,33 +5,33 @@
 #define WG 256
 
 // ---------------------------------------------------------------------------------------
+// Uniform groups and their task regions
+// ---------------------------------------------------------------------------------------
+// A group is up to MAX_GROUP shell pairs of one class, one molecule and one primitive-block
+// count that share their first shell (kernel orientation): kets (c, d_0) .. (c, d_{n-1}) of a
+// ket-keyed class pair, bras (a, b_0) .. of a bra-keyed one.  Its pairs are
+// uorder[first .. first + n).  The screening kernel sweeps the other class once for the whole
+// group; every swept pair that survives against at least one member joins the group's union
+// (in sweep order, at most the class pair's union bound per task region).  One task region
+// (a run of uint2 entries in the class pair's task list) holds
+//   [0]            {n, U}                      members, union size
+//   [1 + m]        {count_m, offset_m}         member m's quartet list (offsets from the region)
+//   [offset_m + t] {swept pair | block << 24, slot}   one entry per primitive block of every
+//                  swept pair whose quartet with member m survives (slot: its union index)
+// so the quartets keep every lane busy (the members' lists run as one dense list), while the J
+// of a swept pair collects in its LDS slot across the members and primitive blocks and leaves
+// the work item once.
+#define MAX_GROUP 16
+struct GroupRec {
+  int first;   // into uorder
+  int n;       // members
+  int ublk;    // primitive blocks of the uniform side (equal for every member)
+  int pad;
+};
+
+// ---------------------------------------------------------------------------------------
 // ERI kernel wrappers, instantiated once per class pair by the generated units
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void make_quartet(const uint2* __restrict__ tasks, long t, long count, long wbase,
-                                             const PairInfo* __restrict__ pairs,
-                                             const PrimPair* __restrict__ prims, QuartetArgs& q) {
-  const bool active = t < count;
-  const uint2 tk = tasks[active ? t : wbase];
-  const unsigned bx = pair_of(tk.x), ky = pair_of(tk.y);
-  const PairInfo B = pairs[bx];
-  const PairInfo K = pairs[ky];
-  int bp0, bp1, kp0, kp1;
-  prim_range(B, block_of(tk.x), bp0, bp1);
-  prim_range(K, block_of(tk.y), kp0, kp1);
-  q.prim = prims;
-  q.bra_p0 = bp0;
-  q.bra_p1 = active ? bp1 : bp0;
-  q.ket_p0 = kp0;
-  q.ket_p1 = kp1;
-  q.a0 = B.a0; q.b0 = B.b0; q.c0 = K.a0; q.d0 = K.b0;
-  const double deg = (B.a == B.b ? 1.0 : 2.0) * (K.a == K.b ? 1.0 : 2.0) * (bx == ky ? 1.0 : 2.0);
-  q.deg = active ? deg : 0.0;
-  const unsigned int k0 = __shfl((int)tk.y, 0);
-  q.ket_uniform = (__ballot(tk.y == k0) == ~0ull);
-  q.bra_swap = B.a != B.s1;
-  q.ket_swap = K.a != K.s1;
-  q.loff = K.loff;
-}
 
 // Quartet of one lane's bra (pair B, task code bra_code) with the wavefront-uniform ket pair
 // kk and its primitive block kblk (both from readfirstlane, so the ket data is loaded through
@@ -56,7 +56,7 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
   q.ket_uniform = true;
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
-  q.loff = K.loff;   // the uniform ket's molecule (every quartet of a segment lies in it)
+  q.loff = K.loff;   // the uniform ket's molecule (every quartet of a group lies in it)
 }
 
 // Quartet of the wavefront-uniform bra pair bb and its primitive block bblk (from
@@ -84,105 +84,126 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
   q.loff = B.loff;   // the uniform bra's molecule
 }
 
-// JK kernel: work items are (chunk, 64-task block) pairs, one chunk per wavefront, so classes
-// whose ket components are split into independent chunks run them in parallel.
-#define DEFINE_CLASS_KERNELS(I, J, BI, KI, NAME, LTOT)                                          \
-  template <int NSLOT>                                                                          \
-  __global__ void __launch_bounds__(WG) eri_jk_##NAME(                                          \
-      const uint2* __restrict__ tasks, const unsigned long long* __restrict__ count_ptr,        \
-      const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                   \
-      const double* __restrict__ boys, JKArgs jk) {                                             \
-    constexpr int NCH = kChunks_##NAME;                                                         \
-    __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                        \
-    const long count = (long)*count_ptr;                                                        \
-    const long nitems = ((count + 63) / 64) * 64 * NCH;                                         \
-    if ((long)blockIdx.x * WG >= nitems) return;                                                \
-    stage_boys(boys, LTOT, boys_lds);                                                           \
-    for (long base = (long)blockIdx.x * WG; base < nitems; base += (long)gridDim.x * WG) {     \
-      const long witem = (base + (threadIdx.x & ~63)) / 64;                                     \
-      const int chunk = (int)(witem % NCH);                                                     \
-      const long tbase = (witem / NCH) * 64;                                                    \
-      if (tbase >= count) break;                                                                \
-      const long t = tbase + (threadIdx.x & 63);                                                \
-      const bool active = t < count;                                                            \
-      const uint2 tk = tasks[active ? t : tbase];                                               \
-      const PairInfo B = pairs[pair_of(tk.x)];                                                  \
-      bool pending = active;                                                                    \
-      /* waterfall over the distinct ket pairs of the wavefront: each pass has one uniform ket \
-         (scalar loads of its primitive data), lanes of other kets contribute nothing */        \
-      while (true) {                                                                            \
-        const unsigned long long pm = __ballot(pending);                                        \
-        if (pm == 0ull) break;                                                                  \
-        const int leader = __ffsll((long long)pm) - 1;                                          \
-        const unsigned kk = __builtin_amdgcn_readfirstlane(__shfl((int)tk.y, leader));          \
-        const bool mine = pending && tk.y == kk;                                                \
-        QuartetArgs q;                                                                          \
-        make_quartet_uniform_ket(B, tk.x, kk, 0u, mine, pairs, prims, q);                       \
-        quartet_jk_##NAME<NSLOT, 0>(q, jk, boys_lds, chunk, nullptr, nullptr);                  \
-        pending = pending && !mine;                                                             \
-      }                                                                                         \
-    }                                                                                           \
-  }                                                                                             \
-  /* Segment kernel (one density slot, WGS threads per workgroup): each workgroup takes     \
-     (segment, chunk) items from a queue; the segment's ket is uniform, K columns c/d and J(3,4) accumulate in LDS and are \
-     flushed to HBM once per item. */                                                        \
-  template <int WGS, typename R>                                                              \
-  __global__ void __launch_bounds__(WGS) eri_jkseg_##NAME(                                     \
+// Take the next work item from the queue (whole workgroup).
+__device__ __forceinline__ long next_item(unsigned* __restrict__ work, int* s_item) {
+  __syncthreads();
+  if (threadIdx.x == 0) *s_item = (int)atomicAdd(work, 1u);
+  __syncthreads();
+  return (long)__builtin_amdgcn_readfirstlane(*s_item);
+}
+
+// Coulomb block (1,2) of a pair held in kernel orientation: address of element (i, j), i a
+// component of its a shell, j of its b shell, in the canonical order (s1, s2).
+__device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
+  const bool sw = P.a != P.s1;
+  return sw ? (long)(P.b0 + j) * N + P.a0 + i : (long)(P.a0 + i) * N + P.b0 + j;
+}
+
+// Group kernels.  Work items are (task region, chunk) pairs from an atomic queue; each workgroup
+// runs the region's members one after the other (the member wavefront-uniform, one quartet per
+// lane), then flushes its LDS accumulators to HBM:
+//  * ket-keyed (eri_grp, DIG 0 or 1): K columns of the shared c shell and of every member's d
+//    shell (DIG 1; with DIG 0 K goes to HBM atomics directly), J(c, d_m) of every member, and J of
+//    every union bra;
+//  * bra-keyed (eri_grpb, DIG 2): K rows of the shared a shell and of every member's b shell,
+//    J(a, b_m) of every member, and J of every union ket.
+// Dynamic LDS, ket-keyed: [(NC + mmax ND) LN K columns (DIG 1)][mmax NC ND J(c,d)][ub NA NB J slots]
+//                         [ub ints: the swept pair of each slot]
+//              bra-keyed: [(NA + mmax NB) LN K rows][mmax NA NB J(a,b)][ub NC ND J slots][ub ints]
+#define DEFINE_CLASS_KERNELS(NAME, LTOT)                                                        \
+  template <int WGS, int DIG, typename R>                                                     \
+  __global__ void __launch_bounds__(WGS) eri_grp_##NAME(                                       \
       const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                      \
       const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                     \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                 \
-      const double* __restrict__ boys, JKArgs jk) {                                           \
-    constexpr int NCH = kChunks_##NAME, NC = kNC_##NAME, ND = kND_##NAME, NCOL = NC + ND;     \
+      const double* __restrict__ boys, JKArgs jk, const GroupRec* __restrict__ groups,        \
+      const int* __restrict__ uorder, int mmax, int ub) {                                     \
+    constexpr int NCH = kChunks_##NAME, NC = kNC_##NAME, ND = kND_##NAME;                     \
+    constexpr int NA = kNA_##NAME, NB = kNB_##NAME, NCD = NC * ND, NAB = NA * NB;             \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                      \
     __shared__ int s_item;                                                                    \
+    __shared__ int s_dcol[MAX_GROUP], s_moff[MAX_GROUP];                                      \
     extern __shared__ double dyn_lds[];                                                       \
     const long nitems = (long)(*nseg_ptr) * NCH;                                              \
     if (nitems == 0) return;                                                                  \
     stage_boys(boys, LTOT, boys_lds);                                                         \
     const int N = jk.ncart, LN = jk.lds_n;                                                    \
+    const int ncolmax = DIG == 1 ? (NC + mmax * ND) * LN : 0;                                 \
     double* colK = dyn_lds;                                                                   \
-    double* jcdL = dyn_lds + NCOL * LN;                                                       \
-    for (int k = threadIdx.x; k < NCOL * LN + NC * ND; k += WGS) dyn_lds[k] = 0.0;            \
+    double* jcdL = dyn_lds + ncolmax;                                                         \
+    double* jsl = jcdL + mmax * NCD;                                                          \
+    int* spair = (int*)(jsl + ub * NAB);                                                      \
+    for (int k = threadIdx.x; k < ncolmax + mmax * NCD + ub * NAB; k += WGS) dyn_lds[k] = 0.0; \
     const int wofs = threadIdx.x & ~63;                                                       \
     while (true) {                                                                            \
-      __syncthreads();                                                                        \
-      if (threadIdx.x == 0) s_item = (int)atomicAdd(work, 1u);                                \
-      __syncthreads();                                                                        \
-      const long item = s_item;                                                               \
+      const long item = next_item(work, &s_item);                                             \
       if (item >= nitems) break;                                                              \
       const Segment sg = segs[item / NCH];                                                    \
       const int chunk = (int)(item % NCH);                                                    \
-      const unsigned kk = __builtin_amdgcn_readfirstlane((unsigned)sg.ket);                   \
+      const GroupRec gr = groups[sg.ket];                                                     \
+      const int n = gr.n;                                                                     \
       const unsigned kblk = __builtin_amdgcn_readfirstlane((unsigned)sg.blk);                 \
-      for (int t0 = 0; t0 < sg.count; t0 += WGS) {                                             \
-        if (t0 + wofs >= sg.count) continue;                                                  \
-        const int t = t0 + threadIdx.x;                                                       \
-        const bool active = t < sg.count;                                                     \
-        const uint2 tk = tasks[sg.off + (active ? t : t0 + wofs)];                            \
-        const PairInfo B = pairs[pair_of(tk.x)];                                              \
-        QuartetArgs q;                                                                        \
-        make_quartet_uniform_ket(B, tk.x, kk, kblk, active, pairs, prims, q);                 \
-        quartet_jk_##NAME<1, 1, R>(q, jk, boys_lds, chunk, colK, jcdL);                       \
+      const uint2* reg = tasks + sg.off;                                                      \
+      const int U = (int)reg[0].y;                                                            \
+      if (threadIdx.x < n) {                                                                  \
+        s_dcol[threadIdx.x] = pairs[uorder[gr.first + threadIdx.x]].b0;                       \
+        s_moff[threadIdx.x] = (int)reg[1 + threadIdx.x].y;                                    \
       }                                                                                       \
       __syncthreads();                                                                        \
-      const PairInfo K = pairs[kk];                                                           \
-      for (int k = threadIdx.x; k < NCOL * LN; k += WGS) {                                     \
-        const double v = colK[k];                                                             \
-        if (v != 0.0) {                                                                       \
-          const int col = k / LN, row = k - col * LN + K.loff;                                \
-          const int gcol = col < NC ? K.a0 + col : K.b0 + (col - NC);                         \
-          atomicAdd(&jk.GK[0][(long)row * N + gcol], v);                                      \
-          colK[k] = 0.0;                                                                      \
+      /* every member's list, one dense run: waterfall over the members of a wavefront */     \
+      const int e0 = 1 + n, E = sg.count - e0;                                                \
+      for (int t0 = 0; t0 < E; t0 += WGS) {                                                   \
+        if (t0 + wofs >= E) continue;                                                         \
+        const int t = t0 + threadIdx.x;                                                       \
+        const bool active = t < E;                                                            \
+        const int pos = e0 + (active ? t : t0 + wofs);                                        \
+        const uint2 e = reg[pos];                                                             \
+        if (active) spair[e.y] = (int)pair_of(e.x);                                           \
+        int mi = 0;                                                                           \
+        for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                      \
+        const PairInfo B = pairs[pair_of(e.x)];                                               \
+        bool pending = active;                                                                \
+        while (true) {                                                                        \
+          const unsigned long long pm = __ballot(pending);                                    \
+          if (pm == 0ull) break;                                                              \
+          const int mu = __builtin_amdgcn_readfirstlane(__shfl(mi, __ffsll((long long)pm) - 1)); \
+          const bool mine = pending && mi == mu;                                              \
+          const unsigned kk = __builtin_amdgcn_readfirstlane((unsigned)uorder[gr.first + mu]); \
+          QuartetArgs q;                                                                      \
+          make_quartet_uniform_ket(B, e.x, kk, kblk, mine, pairs, prims, q);                  \
+          quartet_jk_##NAME<1, DIG, R>(q, jk, boys_lds, chunk, colK, colK + (NC + mu * ND) * LN, \
+                                       jcdL + mu * NCD, jsl + (long)e.y * NAB);                \
+          pending = pending && !mine;                                                         \
         }                                                                                     \
       }                                                                                       \
-      if (threadIdx.x < NC * ND) {                                                            \
-        const double v = jcdL[threadIdx.x];                                                   \
+      __syncthreads();                                                                        \
+      const PairInfo K0 = pairs[uorder[gr.first]];                                            \
+      if constexpr (DIG == 1) {                                                               \
+        for (int k = threadIdx.x; k < (NC + n * ND) * LN; k += WGS) {                          \
+          const double v = colK[k];                                                           \
+          if (v != 0.0) {                                                                     \
+            const int col = k / LN, row = k - col * LN + K0.loff;                             \
+            const int md = (col - NC) / ND;                                                   \
+            const int gcol = col < NC ? K0.a0 + col : s_dcol[md] + (col - NC - md * ND);       \
+            atomicAdd(&jk.GK[0][(long)row * N + gcol], v);                                    \
+            colK[k] = 0.0;                                                                    \
+          }                                                                                   \
+        }                                                                                     \
+      }                                                                                       \
+      for (int k = threadIdx.x; k < n * NCD; k += WGS) {                                      \
+        const double v = jcdL[k];                                                             \
         if (v != 0.0) {                                                                       \
-          const bool ks = K.a != K.s1;                                                        \
-          const int ic = threadIdx.x / ND, idd = threadIdx.x - (threadIdx.x / ND) * ND;       \
-          const int tcd = ks ? K.b0 * N + K.a0 : K.a0 * N + K.b0;                             \
-          atomicAdd(&jk.GJ[0][tcd + ic * (ks ? 1 : N) + idd * (ks ? N : 1)], v);              \
-          jcdL[threadIdx.x] = 0.0;                                                            \
+          const int m = k / NCD, ic = (k - m * NCD) / ND, idd = k - m * NCD - ic * ND;        \
+          atomicAdd(&jk.GJ[0][jaddr(pairs[uorder[gr.first + m]], N, ic, idd)], v);            \
+          jcdL[k] = 0.0;                                                                      \
+        }                                                                                     \
+      }                                                                                       \
+      for (int k = threadIdx.x; k < U * NAB; k += WGS) {                                      \
+        const double v = jsl[k];                                                              \
+        if (v != 0.0) {                                                                       \
+          const int u = k / NAB, ia = (k - u * NAB) / NB, ib = k - u * NAB - ia * NB;         \
+          atomicAdd(&jk.GJ[0][jaddr(pairs[spair[u]], N, ia, ib)], v);                         \
+          jsl[k] = 0.0;                                                                       \
         }                                                                                     \
       }                                                                                       \
     }                                                                                         \
@@ -200,96 +221,129 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
       const long wbase = base + (threadIdx.x & ~63);                                            \
       if (wbase >= count) break;                                                                \
       const long t = base + threadIdx.x;                                                        \
+      const bool active = t < count;                                                            \
+      const uint2 tk = tasks[active ? t : wbase];                                               \
+      const PairInfo B = pairs[pair_of(tk.x)];                                                  \
+      const PairInfo K = pairs[pair_of(tk.y)];                                                  \
       QuartetArgs q;                                                                            \
-      make_quartet(tasks, t, count, wbase, pairs, prims, q);                                    \
-      if (t < count) quartet_block_##NAME(q, out + (slot ? (long)slot[t] : t) * (long)blk, boys_lds); \
+      q.prim = prims;                                                                           \
+      prim_range(B, block_of(tk.x), q.bra_p0, q.bra_p1);                                        \
+      prim_range(K, block_of(tk.y), q.ket_p0, q.ket_p1);                                        \
+      if (active) quartet_block_##NAME(q, out + (slot ? (long)slot[t] : t) * (long)blk, boys_lds); \
     }                                                                                           \
   }
 
-// Bra-uniform segment kernel (one density slot), for class pairs whose bra class has few
-// pairs: a segment is one bra pair with the ket pairs that survive against it.  K rows a/b
-// (NA + NB rows of ncart) and J(1,2) accumulate in LDS and are flushed once per item; J(3,4)
-// goes to HBM per lane.
+// Bra-keyed group kernel (DIG 2), for class pairs whose bra class has few pairs: the uniform
+// members are bras (a, b_m) sharing shell a, the lanes hold the kets that survive against them.
 #define DEFINE_CLASS_SEGB(NAME, LTOT)                                                           \
   template <typename R>                                                                         \
-  __global__ void __launch_bounds__(WG) eri_jksegb_##NAME(                                      \
+  __global__ void __launch_bounds__(WG) eri_grpb_##NAME(                                        \
       const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                        \
       const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                       \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                   \
-      const double* __restrict__ boys, JKArgs jk) {                                             \
-    constexpr int NCH = kChunks_##NAME, NA = kNA_##NAME, NB = kNB_##NAME, NROW = NA + NB;       \
+      const double* __restrict__ boys, JKArgs jk, const GroupRec* __restrict__ groups,          \
+      const int* __restrict__ uorder, int mmax, int ub) {                                       \
+    constexpr int NCH = kChunks_##NAME, NC = kNC_##NAME, ND = kND_##NAME;                       \
+    constexpr int NA = kNA_##NAME, NB = kNB_##NAME, NCD = NC * ND, NAB = NA * NB;               \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                        \
     __shared__ int s_item;                                                                      \
+    __shared__ int s_brow[MAX_GROUP], s_moff[MAX_GROUP];                                        \
     extern __shared__ double dyn_lds[];                                                         \
     const long nitems = (long)(*nseg_ptr) * NCH;                                                \
     if (nitems == 0) return;                                                                    \
     stage_boys(boys, LTOT, boys_lds);                                                           \
     const int N = jk.ncart, LN = jk.lds_n;                                                      \
+    const int nrowmax = (NA + mmax * NB) * LN;                                                  \
     double* rowK = dyn_lds;                                                                     \
-    double* jabL = dyn_lds + NROW * LN;                                                         \
-    for (int k = threadIdx.x; k < NROW * LN + NA * NB; k += WG) dyn_lds[k] = 0.0;              \
+    double* jabL = dyn_lds + nrowmax;                                                           \
+    double* jsl = jabL + mmax * NAB;                                                            \
+    int* spair = (int*)(jsl + ub * NCD);                                                        \
+    for (int k = threadIdx.x; k < nrowmax + mmax * NAB + ub * NCD; k += WG) dyn_lds[k] = 0.0;   \
     const int wofs = threadIdx.x & ~63;                                                         \
     while (true) {                                                                              \
-      __syncthreads();                                                                          \
-      if (threadIdx.x == 0) s_item = (int)atomicAdd(work, 1u);                                  \
-      __syncthreads();                                                                          \
-      const long item = s_item;                                                                 \
+      const long item = next_item(work, &s_item);                                               \
       if (item >= nitems) break;                                                                \
       const Segment sg = segs[item / NCH];                                                      \
       const int chunk = (int)(item % NCH);                                                      \
-      const unsigned bb = __builtin_amdgcn_readfirstlane((unsigned)sg.ket);                     \
+      const GroupRec gr = groups[sg.ket];                                                       \
+      const int n = gr.n;                                                                       \
       const unsigned bblk = __builtin_amdgcn_readfirstlane((unsigned)sg.blk);                   \
-      for (int t0 = 0; t0 < sg.count; t0 += WG) {                                               \
-        if (t0 + wofs >= sg.count) continue;                                                    \
-        const int t = t0 + threadIdx.x;                                                         \
-        const bool active = t < sg.count;                                                       \
-        const uint2 tk = tasks[sg.off + (active ? t : t0 + wofs)];                              \
-        const PairInfo K = pairs[pair_of(tk.y)];                                                \
-        QuartetArgs q;                                                                          \
-        make_quartet_uniform_bra(K, tk.y, bb, bblk, active, pairs, prims, q);                   \
-        quartet_jk_##NAME<1, 2, R>(q, jk, boys_lds, chunk, rowK, jabL);                         \
+      const uint2* reg = tasks + sg.off;                                                        \
+      const int U = (int)reg[0].y;                                                              \
+      if (threadIdx.x < n) {                                                                    \
+        s_brow[threadIdx.x] = pairs[uorder[gr.first + threadIdx.x]].b0;                         \
+        s_moff[threadIdx.x] = (int)reg[1 + threadIdx.x].y;                                      \
       }                                                                                         \
       __syncthreads();                                                                          \
-      const PairInfo A = pairs[bb];                                                             \
-      for (int k = threadIdx.x; k < NROW * LN; k += WG) {                                       \
+      const int e0 = 1 + n, E = sg.count - e0;                                                  \
+      for (int t0 = 0; t0 < E; t0 += WG) {                                                      \
+        if (t0 + wofs >= E) continue;                                                           \
+        const int t = t0 + threadIdx.x;                                                         \
+        const bool active = t < E;                                                              \
+        const int pos = e0 + (active ? t : t0 + wofs);                                          \
+        const uint2 e = reg[pos];                                                               \
+        if (active) spair[e.y] = (int)pair_of(e.x);                                             \
+        int mi = 0;                                                                             \
+        for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                        \
+        const PairInfo K = pairs[pair_of(e.x)];                                                 \
+        bool pending = active;                                                                  \
+        while (true) {                                                                          \
+          const unsigned long long pm = __ballot(pending);                                      \
+          if (pm == 0ull) break;                                                                \
+          const int mu = __builtin_amdgcn_readfirstlane(__shfl(mi, __ffsll((long long)pm) - 1)); \
+          const bool mine = pending && mi == mu;                                                \
+          const unsigned bb = __builtin_amdgcn_readfirstlane((unsigned)uorder[gr.first + mu]);  \
+          QuartetArgs q;                                                                        \
+          make_quartet_uniform_bra(K, e.x, bb, bblk, mine, pairs, prims, q);                    \
+          quartet_jk_##NAME<1, 2, R>(q, jk, boys_lds, chunk, rowK, rowK + (NA + mu * NB) * LN,   \
+                                     jabL + mu * NAB, jsl + (long)e.y * NCD);                   \
+          pending = pending && !mine;                                                           \
+        }                                                                                       \
+      }                                                                                         \
+      __syncthreads();                                                                          \
+      const PairInfo A0 = pairs[uorder[gr.first]];                                              \
+      for (int k = threadIdx.x; k < (NA + n * NB) * LN; k += WG) {                              \
         const double v = rowK[k];                                                               \
         if (v != 0.0) {                                                                         \
-          const int lrow = k / LN, col = k - lrow * LN + A.loff;                                \
-          const int grow = lrow < NA ? A.a0 + lrow : A.b0 + (lrow - NA);                        \
+          const int lrow = k / LN, col = k - lrow * LN + A0.loff;                               \
+          const int mb = (lrow - NA) / NB;                                                      \
+          const int grow = lrow < NA ? A0.a0 + lrow : s_brow[mb] + (lrow - NA - mb * NB);        \
           atomicAdd(&jk.GK[0][(long)grow * N + col], v);                                        \
           rowK[k] = 0.0;                                                                        \
         }                                                                                       \
       }                                                                                         \
-      if (threadIdx.x < NA * NB) {                                                              \
-        const double v = jabL[threadIdx.x];                                                     \
+      for (int k = threadIdx.x; k < n * NAB; k += WG) {                                         \
+        const double v = jabL[k];                                                               \
         if (v != 0.0) {                                                                         \
-          const bool bs = A.a != A.s1;                                                          \
-          const int ia = threadIdx.x / NB, ib = threadIdx.x - (threadIdx.x / NB) * NB;          \
-          const int tab = bs ? A.b0 * N + A.a0 : A.a0 * N + A.b0;                               \
-          atomicAdd(&jk.GJ[0][tab + ia * (bs ? 1 : N) + ib * (bs ? N : 1)], v);                 \
-          jabL[threadIdx.x] = 0.0;                                                              \
+          const int m = k / NAB, ia = (k - m * NAB) / NB, ib = k - m * NAB - ia * NB;           \
+          atomicAdd(&jk.GJ[0][jaddr(pairs[uorder[gr.first + m]], N, ia, ib)], v);               \
+          jabL[k] = 0.0;                                                                        \
+        }                                                                                       \
+      }                                                                                         \
+      for (int k = threadIdx.x; k < U * NCD; k += WG) {                                         \
+        const double v = jsl[k];                                                                \
+        if (v != 0.0) {                                                                         \
+          const int u = k / NCD, ic = (k - u * NCD) / ND, idd = k - u * NCD - ic * ND;          \
+          atomicAdd(&jk.GJ[0][jaddr(pairs[spair[u]], N, ic, idd)], v);                          \
+          jsl[k] = 0.0;                                                                         \
         }                                                                                       \
       }                                                                                         \
     }                                                                                           \
   }
 
-typedef void (*JKKernel)(const uint2*, const unsigned long long*, const PairInfo*, const PrimPair*,
-                         const double*, JKArgs);
 typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int,
                             const int*);
-typedef void (*SegKernel)(const uint2*, const Segment*, const unsigned*, unsigned*, const PairInfo*, const PrimPair*,
-                          const double*, JKArgs);
+typedef void (*GroupKernel)(const uint2*, const Segment*, const unsigned*, unsigned*, const PairInfo*,
+                            const PrimPair*, const double*, JKArgs, const GroupRec*, const int*, int, int);
 
 struct ClassKernels {
   int i, j, bi, ki, L, nchunk;
   const char* name;
-  JKKernel jk1, jk2;
   BlockKernel block;
-  SegKernel jkseg;
-  SegKernel jksegb;   // bra-uniform segment kernel, or null
+  // grp[precision][variant]: precision 0 FP64 / 1 FP32 integrals; variant 0 ket-keyed with HBM K
+  // atomics, 1 ket-keyed with LDS K columns, 2 bra-keyed with LDS K rows (null where the class
+  // pair has no bra-keyed form)
+  GroupKernel grp[2][3];
   int nc, nd, na, nb;
-  int seg_wg;         // workgroup size of jkseg
-  SegKernel jkseg32, jksegb32;   // the same with FP32 integrals (mixed precision)
+  int seg_wg;         // workgroup size of the ket-keyed kernels
 };
-
-

@@ -213,13 +213,17 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
     em.ind = 0
     em(f"// ---- class ({name[0]}{name[1]}|{name[2]}{name[3]}) kind={kind} chunks={len(chunks)} persist={persist}")
     if kind == "jk":
-        # DIG: 0 = every J/K update is an HBM atomic; 1 = the ket is wavefront-uniform: K columns
-        # (c, d) accumulate in LDS (colK) and J(3,4) is reduced across the wavefront into LDS (jcdL);
-        # 2 = the bra is wavefront-uniform: K rows (a, b) accumulate in LDS (colK holds the rows)
-        # and J(1,2) is reduced into LDS (jcdL holds J(a,b)), J(3,4) is a per-lane HBM atomic.
+        # The wavefront's lanes share one uniform pair (the ket for DIG 0 and 1, the bra for DIG 2);
+        # each lane holds one quartet with its own swept pair.
+        # DIG 0: K updates are HBM atomics; DIG 1 (uniform ket): K columns c and d accumulate in LDS
+        # (colC: the c columns, colD: the d columns, LN rows each); DIG 2 (uniform bra): K rows a
+        # and b accumulate in LDS (colC: the a rows, colD: the b rows, LN columns each).
+        # J of the uniform pair is reduced across the wavefront into LDS (jcdL); J of the lane's
+        # own swept pair is added to its LDS slot jl (shared by the work item's quartets of that
+        # pair, flushed once per item).
         # R: arithmetic type of the integrals and their digestion (LDS and HBM sums stay double)
         em("template <int NSLOT, int DIG, typename R = double>")
-        em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds, int chunk, double* colK, double* jcdL) {{")
+        em(f"__device__ __forceinline__ void quartet_jk_{name}(const QuartetArgs& q, const JKArgs& jk, const double* __restrict__ boys_lds, int chunk, double* colC, double* colD, double* jcdL, double* jl) {{")
     else:
         em(f"__device__ __forceinline__ void quartet_block_{name}(const QuartetArgs& q, double* __restrict__ out, const double* __restrict__ boys_lds) {{")
     em.ind = 1
@@ -416,14 +420,13 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
                 em(f"  jcd += o_{ia}_{ib}_{ic}_{idd} * R(D[{_ab('xba', ia, ib)}]);")
                 tgt = f"jab_{ia}_{ib}[s]" if persist else f"jab_{ia}_{ib}"
                 em(f"  {tgt} += o_{ia}_{ib}_{ic}_{idd} * x43;")
-        em(f"  if constexpr (DIG == 1) wave_add_lds(&jcdL[s * {NC * ND} + {ic * ND + idd}], double(dg * jcd));")
-        em(f"  else if constexpr (DIG == 2) {{ if (live) atomicAdd(&jk.GJ[s][{_cd('tcd', ic, idd)}], double(dg * jcd)); }}")
-        em(f"  else wave_add(&jk.GJ[s][{_cd('tcd', ic, idd)}], double(dg * jcd), q.ket_uniform); }}")
+        em(f"  if constexpr (DIG == 2) {{ if (live) atomicAdd(&jl[{ic * ND + idd}], double(dg * jcd)); }}")
+        em(f"  else wave_add_lds(&jcdL[{ic * ND + idd}], double(dg * jcd)); }}")
     if not persist:
         for ia in range(NA):
             for ib in range(NB):
-                em(f"if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], double(dg * jab_{ia}_{ib}));")
-                em(f"else if (live) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], double(dg * jab_{ia}_{ib}));")
+                em(f"if constexpr (DIG == 2) wave_add_lds(&jcdL[{ia * NB + ib}], double(dg * jab_{ia}_{ib}));")
+                em(f"else if (live) atomicAdd(&jl[{ia * NB + ib}], double(dg * jab_{ia}_{ib}));")
     em("if (ns) {")
     em.ind += 1
     for (ic, idd) in ks:
@@ -456,10 +459,10 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
         glob = f"atomicAdd(&{G}[(q.{row0} + {ri}) * N + {col0} + {ci}], double({val}))"
         if trans:
             return f"if (live) {glob};"
-        lcol = ci if colkind == "c" else NC + ci
-        lds = f"atomicAdd(&colK[(s * {NCOL} + {lcol}) * LN + q.{row0} - q.loff + {ri}], double({val}))"
-        lrow = ri if row0 == "a0" else NA + ri
-        ldsb = f"atomicAdd(&colK[(s * {NA + NB} + {lrow}) * LN + {col0} - q.loff + {ci}], double({val}))"
+        colp = "colC" if colkind == "c" else "colD"
+        lds = f"atomicAdd(&{colp}[{ci} * LN + q.{row0} - q.loff + {ri}], double({val}))"
+        rowp = "colC" if row0 == "a0" else "colD"
+        ldsb = f"atomicAdd(&{rowp}[{ri} * LN + {col0} - q.loff + {ci}], double({val}))"
         return (f"if (live) {{ if constexpr (DIG == 1) {lds}; else if constexpr (DIG == 2) {ldsb}; "
                 f"else {glob}; }}")
 
@@ -543,19 +546,19 @@ def gen_digest_final(em: Emitter, NA, NB, NC, ND):
     em("if (jk.do_j[s]) {")
     for ia in range(NA):
         for ib in range(NB):
-            em(f"  if constexpr (DIG == 2) wave_add_lds(&jcdL[s * {NA * NB} + {ia * NB + ib}], double(dg * jab_{ia}_{ib}[s]));")
-            em(f"  else if (q.deg != 0.0) atomicAdd(&jk.GJ[s][{_ab('tab', ia, ib)}], double(dg * jab_{ia}_{ib}[s]));")
+            em(f"  if constexpr (DIG == 2) wave_add_lds(&jcdL[{ia * NB + ib}], double(dg * jab_{ia}_{ib}[s]));")
+            em(f"  else if (q.deg != 0.0) atomicAdd(&jl[{ia * NB + ib}], double(dg * jab_{ia}_{ib}[s]));")
     em("}")
     em("if (q.deg != 0.0 && jk.do_k[s]) {")
     for ia in range(NA):
         for idd in range(ND):
-            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * LN + q.a0 - q.loff + {ia}], double(hk * kad_{ia}_{idd}[s]));")
-            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {ia}) * LN + q.d0 - q.loff + {idd}], double(hk * kad_{ia}_{idd}[s]));")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colD[{idd} * LN + q.a0 - q.loff + {ia}], double(hk * kad_{ia}_{idd}[s]));")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colC[{ia} * LN + q.d0 - q.loff + {idd}], double(hk * kad_{ia}_{idd}[s]));")
             em(f"  else atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], double(hk * kad_{ia}_{idd}[s]));")
     for ib in range(NB):
         for idd in range(ND):
-            em(f"  if constexpr (DIG == 1) atomicAdd(&colK[(s * {NCOL} + {NC + idd}) * LN + q.b0 - q.loff + {ib}], double(hk * kbd_{ib}_{idd}[s]));")
-            em(f"  else if constexpr (DIG == 2) atomicAdd(&colK[(s * {NA + NB} + {NA + ib}) * LN + q.d0 - q.loff + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colD[{idd} * LN + q.b0 - q.loff + {ib}], double(hk * kbd_{ib}_{idd}[s]));")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colD[{ib} * LN + q.d0 - q.loff + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
             em(f"  else atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
     em("}")
     em.ind -= 1
@@ -598,21 +601,22 @@ def main(out_dir: str, maxl: int = 2):
                    '#include "../gdirect_kernels.h"', ""]
             src.append(gen_class(LA, LB, LC, LD, "jk"))
             src.append(gen_class(LA, LB, LC, LD, "block"))
-            src.append(f"DEFINE_CLASS_KERNELS({i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {name}, {L})")
-            # bra-uniform segments pay off where the bra class has far fewer pairs than the ket
+            src.append(f"DEFINE_CLASS_KERNELS({name}, {L})")
+            # bra-uniform groups pay off where the bra class has far fewer pairs than the ket
             # class (few bras survive per ket): different pair classes and a bra above (ss)
             segb = i != j and (LA, LB) != (0, 0)
             if segb:
                 src.append(f"DEFINE_CLASS_SEGB({name}, {L})")
-            # ket-keyed segments of an (ss) or (ps) bra class hold many bras per ket: 512-thread
+            # ket-keyed groups of an (ss) or (ps) bra class hold many bras per ket: 512-thread
             # workgroups share one Boys table and one set of K columns over 8 wavefronts
             seg_wg = 512 if (LA, LB) in ((0, 0), (1, 0)) else 256
+            gb = (lambda R: f"eri_grpb_{name}<{R}>") if segb else (lambda R: "nullptr")
             src.append(f"void cq_register_{name}(ClassKernels& ck) {{")
             src.append(f"  ck = ClassKernels{{{i}, {j}, {pcl.index(bra)}, {pcl.index(ket)}, {L}, kChunks_{name}, \"{name}\", "
-                       f"eri_jk_{name}<1>, eri_jk_{name}<2>, eri_block_{name}, eri_jkseg_{name}<{seg_wg}, double>, "
-                       f"{('eri_jksegb_' + name + '<double>') if segb else 'nullptr'}, kNC_{name}, kND_{name}, "
-                       f"kNA_{name}, kNB_{name}, {seg_wg}, eri_jkseg_{name}<{seg_wg}, float>, "
-                       f"{('eri_jksegb_' + name + '<float>') if segb else 'nullptr'}}};")
+                       f"eri_block_{name}, "
+                       f"{{{{eri_grp_{name}<{seg_wg}, 0, double>, eri_grp_{name}<{seg_wg}, 1, double>, {gb('double')}}}, "
+                       f"{{eri_grp_{name}<{seg_wg}, 0, float>, eri_grp_{name}<{seg_wg}, 1, float>, {gb('float')}}}}}, "
+                       f"kNC_{name}, kND_{name}, kNA_{name}, kNB_{name}, {seg_wg}}};")
             src.append("}")
             _write_if_changed(os.path.join(out_dir, f"eri_{name}.hip"), "\n".join(src) + "\n")
             table.append(name)

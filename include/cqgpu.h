@@ -149,7 +149,7 @@ int cq_set_launch_streams(cq_ctx_t* ctx, int n);
  * and computes quartet by quartet on the host, aointegrals_contract.h:130-146).  The largest
  * task list of one class pair, in 8-byte entries (1 .. 2^31-1, default 2^28 = 2 GiB); a class
  * pair whose worst case (every quartet of its pairs passing the screen) is larger is screened
- * and digested in batches of its uniform pairs, one after another, so memory stays bounded
+ * and digested in batches of its uniform groups, one after another, so memory stays bounded
  * whatever the system size.  Waits for the context's work and reallocates.  Results agree to
  * rounding for every budget. */
 int cq_set_task_budget(cq_ctx_t* ctx, long long entries);
@@ -157,11 +157,28 @@ int cq_set_task_budget(cq_ctx_t* ctx, long long entries);
  * takes. */
 int cq_get_task_plan(const cq_ctx_t* ctx, long long* entries, int* max_batches);
 
+/* Tile of class pair k's digestion kernels (no reference counterpart; set by the autotuner).
+ * A work item digests one group of up to 16 uniform shell pairs sharing a shell against the
+ * swept pairs that survive the screen against any of them; the J of each swept pair collects in
+ * an LDS slot across the group (one HBM update per item instead of one per quartet).
+ * lds_bytes: dynamic LDS budget of the kernel (0: 48 KiB); max_group: cap on the group size,
+ * 1..16, within the budget (0: the default, single pairs); union_bound: swept pairs per task
+ * region, 64..1024 (0: the class pair's default).  Waits for the context's work and re-plans.  Results agree to rounding for
+ * every tile. */
+int cq_set_class_tile(cq_ctx_t* ctx, int k, long long lds_bytes, int max_group, int union_bound);
+/* The planned tile for digestion variant 0/1 (ket-keyed) or 2 (bra-keyed): largest group, union
+ * bound, number of groups, dynamic LDS bytes of the kernel. */
+int cq_get_class_tile(const cq_ctx_t* ctx, int k, int variant, int* max_group, int* union_bound, int* ngroups,
+                      long long* lds_bytes);
+
 /* Profiling (no reference counterpart beyond its wall-clock G[P] timer,
  * include/singleslater/singleslater_fock.h:116-119): HIP events around every class-pair
  * kernel; accumulated milliseconds and launch counts per class pair. */
 int cq_set_profiling(cq_ctx_t* ctx, int on);
 int cq_get_kernel_times(cq_ctx_t* ctx, double* ms, long long* launches, int max_n);
+/* The same for each class pair's screening kernel (the canonical quartet loop and the
+ * density-weighted Schwarz test, aointegrals_contract.h:96-131). */
+int cq_get_screen_times(cq_ctx_t* ctx, double* ms, long long* launches, int max_n);
 
 /* Work sums of the last contraction per class pair, 4 doubles each: sum K_ab*K_cd
  * (primitive quartets), sum K_ab, Cartesian->spherical transform FLOPs, output integrals.

@@ -56,6 +56,8 @@ void newTwoEContractDirect(cq_ctx_t* ctx, const std::vector<std::reference_wrapp
   const size_t nMat = X.size();
   if (AX.size() != nMat || contractionList.size() != nMat || scalingFactors.size() != nMat)
     throw std::runtime_error("cqgpu: X, AX, contractionList and scalingFactors differ in length");
+  // an empty list contracts nothing (the reference's loops do not run)
+  if (nMat == 0) return;
   std::vector<const double*> xs(nMat);
   std::vector<double*> axs(nMat);
   std::vector<int> types(nMat);

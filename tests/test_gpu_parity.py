@@ -255,8 +255,8 @@ def test_per_class_digestion_variants_match_oracle():
     assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL
     assert sum(ao.quartet_counts().values()) == n
     choice = ao.autotune(torch.from_numpy(np.ascontiguousarray(X)).cuda(), reps=1)
-    assert len(choice) == nc and all(v in (0, 1, 2) for v, _ in choice.values())
-    assert ao.class_digestion() == [v for v, _ in choice.values()]
+    assert len(choice) == nc and all(v in (0, 1, 2) for v, _, _ in choice.values())
+    assert ao.class_digestion() == [v for v, _, _ in choice.values()]
     J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
     assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL
     with pytest.raises(_lib.CQError):
@@ -321,3 +321,30 @@ def test_large_basis_lds_segments_match_hbm_digestion():
     assert n2 == n0
     assert float((J2 - J0).abs().max()) < 1e-9 * max(1.0, float(J0.abs().max()))
     assert float((K2 - K0).abs().max()) < 1e-9 * max(1.0, float(K0.abs().max()))
+
+
+@pytest.mark.parametrize("max_group,union_bound,lds", [(1, 64, 0), (4, 256, 0), (16, 1024, 160 * 1024), (8, 128, 24 * 1024)])
+def test_group_tiles_match_oracle(max_group, union_bound, lds):
+    """Groups of up to 16 uniform pairs sharing a shell, task regions of 64 .. 1024 swept pairs
+    (J collected in LDS slots across a group), in every digestion variant: J and K against the
+    oracle with identical surviving-quartet counts ((H2O)8 / cc-pVDZ)."""
+    from chronusq_public_amd import _lib
+    L = _lib.load()
+    mol = M.water_cluster(8, seed=11, box_ang=M.scaled_box(8))
+    bs, ao, ob = _setup(mol, "cc-pvdz")
+    X = _sym_random(bs.nbasis, 13)
+    Qg = ao.computeSchwartz()
+    (Jo, Ko), n = _oracle_jk(ob, Qg, [X, X], [0, 1], [1.0, -0.5], 1e-12)
+    nc = L.cq_num_class_pairs()
+    for k in range(nc):
+        ao.set_class_tile(k, max_group, union_bound, lds)
+    assert max(ao.class_tile(k)[0] for k in range(nc)) > 1 or max_group == 1
+    for lds_mode in (2, 1, 0):
+        _lib.check(L.cq_set_lds_digestion(ao._h, lds_mode), ao._h)
+        J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+        assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL, (lds_mode, max_group, union_bound)
+        assert sum(ao.quartet_counts().values()) == n
+    with pytest.raises(_lib.CQError):
+        ao.set_class_tile(0, 17, 0, 0)
+    with pytest.raises(_lib.CQError):
+        ao.set_class_tile(0, 1, 32, 0)
