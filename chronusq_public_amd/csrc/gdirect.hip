@@ -81,6 +81,33 @@ static size_t device_dyn_lds() {
 static const int kNumClassPairs = CQ_NUM_CLASS_PAIRS;
 static const bool kClassKernelsReady = (cq_register_all(kClassKernels), true);
 
+// Per-call workspace of cq_contract_direct_device_noalloc: the Cartesian density and the four
+// J/K accumulators of every density slot, then the slots' symmetry flags.  One copy per device,
+// held until the process ends; every call on a device shares it, so calls that overlap on one
+// device race.  Host code never uses the array's name as an address: the device address comes
+// from hipGetSymbolAddress, resolved once per device at context creation (scratch_address).
+#define CQ_NOALLOC_SCRATCH_BYTES (64ull << 20)
+static __device__ __align__(256) double scratch[CQ_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;   // by device id; null until resolved
+
+// Device address of `scratch` on the current device (resolved on first use per device).
+static double* scratch_address() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 // ---------------------------------------------------------------------------------------
 // Screening: one wavefront per uniform group; task regions (layout in gdirect_kernels.h).
 // ---------------------------------------------------------------------------------------
@@ -738,6 +765,7 @@ struct cq_ctx {
   // per class pair: the digestion table of the last contraction (work statistics)
   std::vector<int> last_table = std::vector<int>(CQ_NUM_CLASS_PAIRS, 0);
   bool plan_ok = false;   // task and segment buffers are allocated for the current plan
+  double* scratch = nullptr;   // this device's address of the no-allocation workspace
   std::vector<std::vector<int>> batch_start[2];
   std::vector<unsigned char> nblk_h, nublk_h;
   std::vector<int> pair_mol_h;
@@ -1322,6 +1350,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
                      });
   c->d_uorder.upload(c->uorder_h, c->stream);
   plan_tasks(c);
+  c->scratch = scratch_address();
   c->d_nseg.ensure(kNumClassPairs);
   c->d_work_q.ensure((size_t)kNumClassPairs * MAX_MATS);   // one work queue per class pair and slot pass
   c->d_counts.ensure(kNumClassPairs);
@@ -1349,8 +1378,10 @@ struct Slot {
 };
 
 // The contraction on device-resident X/AX (column-major, leading dimension ld).
+// With noalloc, the per-call workspace is the device's static scratch array (nothing is
+// allocated: a call whose workspace exceeds it is rejected) and no profiling events are recorded.
 void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const* AX, const int* types,
-                     const double* scales, int n, int ld, hipStream_t s) {
+                     const double* scales, int n, int ld, hipStream_t s, bool noalloc = false) {
   if (nmat < 1 || nmat > MAX_MATS) throw std::invalid_argument("nmat must be in 1..8");
   if (!c->plan_ok) throw std::runtime_error("no task buffers: the last cq_set_task_budget could not allocate a plan");
   if (n != c->nsph) throw std::invalid_argument("matrix dimension does not match the basis");
@@ -1380,11 +1411,23 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   const int nslot = (int)slots.size();
   // workspace: per slot Dc, GJ, GJ2, GK, GK2, then one flag per slot
   const size_t work_words = (size_t)std::max(1, nslot) * 5 * nc2;
-  c->d_work.ensure(work_words);
-  c->d_flags.ensure(std::max(1, nslot));
-  double* work = c->d_work.p;
-  int* flags = c->d_flags.p;
-  const bool profiling = c->profiling;
+  double* work = nullptr;
+  int* flags = nullptr;
+  if (noalloc) {
+    const size_t need = work_words * sizeof(double) + (size_t)std::max(1, nslot) * sizeof(int);
+    if (need > CQ_NOALLOC_SCRATCH_BYTES)
+      throw std::invalid_argument("the no-allocation workspace (64 MiB) is too small for this call: it needs " +
+                                  std::to_string(need) + " bytes (5 ncart^2 doubles per distinct density)");
+    if (!c->scratch) throw std::runtime_error("no-allocation workspace not resolved on this device");
+    work = c->scratch;
+    flags = (int*)(c->scratch + work_words);
+  } else {
+    c->d_work.ensure(work_words);
+    c->d_flags.ensure(std::max(1, nslot));
+    work = c->d_work.p;
+    flags = c->d_flags.p;
+  }
+  const bool profiling = c->profiling && !noalloc;
   // 1. one launch zeroes the accumulators, counters and flags
   ZeroList zl{};
   auto zero = [&](void* p, size_t bytes) {
@@ -1790,6 +1833,14 @@ int cq_contract_direct_device(cq_ctx_t* c, int nmat, const double* const* dX, do
   return guarded(c, [&] {
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     contract_device(c, nmat, dX, dAX, types, scales, n, ld, s);
+  });
+}
+
+int cq_contract_direct_device_noalloc(cq_ctx_t* c, int nmat, const double* const* dX, double* const* dAX,
+                                      const int* types, const double* scales, int n, int ld, void* stream) {
+  return guarded(c, [&] {
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    contract_device(c, nmat, dX, dAX, types, scales, n, ld, s, true);
   });
 }
 

@@ -100,6 +100,17 @@ int cq_contract_direct(cq_ctx_t* ctx, int nmat, const double* const* X, double* 
 int cq_contract_direct_device(cq_ctx_t* ctx, int nmat, const double* const* X, double* const* AX,
                               const int* type, const double* scale, int n, int ld, void* stream);
 
+/* cq_contract_direct_device without any device allocation (no reference counterpart; for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops).  The per-call
+ * workspace -- 5 ncart^2 doubles per distinct density with a non-zero scale, plus one int each --
+ * lives in a 64 MiB device array of the library, one per device for the life of the process; a
+ * call that needs more returns CQ_ERR_INVALID.  Every call on a device shares that array: calls
+ * that overlap on one device (concurrent streams or threads) race.  The context's own buffers
+ * (task lists, counters) are allocated at cq_ctx_create / cq_set_task_budget, never here.
+ * Profiling (cq_set_profiling) records no events on this path. */
+int cq_contract_direct_device_noalloc(cq_ctx_t* ctx, int nmat, const double* const* X, double* const* AX,
+                                      const int* type, const double* scale, int n, int ld, void* stream);
+
 /* One-electron integrals S, T, V in the shells' basis, column-major n x n (leading dimension
  * ld) in device memory; Z[natom] and xyz[3*natom] (bohr) on the host.  Replaces
  * AOIntegrals::computeAOOneE (src/aointegrals/aointegrals_builders.cpp:337-467).  Synchronous. */
