@@ -200,19 +200,32 @@ def main():
     SingleSlater(M.water_cluster(2, seed=1, box_ang=M.scaled_box(2)), "cc-pvdz", tau=args.tau,
                  guess="LEARNED").SCF3()
 
-    def run_scf(inc, cold, precision="fp64", guess="SAD", judge=True):
+    def run_scf(inc, cold, precision="fp64", guess="SAD", judge="auto", profile=False):
         if cold:
             clear_sad_cache()   # the SAD guess recomputes its atomic UHF densities
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r = SingleSlater(mol, bs, tau=args.tau, guess=guess, incremental=inc, rebuild_every=8,
-                         precision=precision, mixed_threshold=1e-5, judge_guess=judge).SCF3()
+                         precision=precision, mixed_threshold=1e-5, judge_guess=judge).SCF3(profile=profile)
         torch.cuda.synchronize()
         return r, time.perf_counter() - t0
 
     r, wall = run_scf(False, True)
     E_ref, P_conv = r.energy, r.P
-    scf = {}
+    # the oracle's FP64 energy of this cluster (tests/golden/scf_oracle.json, seed 0 only)
+    E_oracle = None
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "scf_oracle.json")) as f:
+            if args.seed == 0:
+                E_oracle = json.load(f)["molecules"]["h2o16_ccpvdz_seed0"]["E"]
+    except (OSError, KeyError):
+        pass
+    scf = {"E_fp64_device": E_ref, "E_fp64_oracle": E_oracle,
+           "note": "dE_vs_oracle against the oracle's FP64 SCF energy; time_to_1e-6 against the device FP64 "
+                   "energy (equal to the oracle's within 1e-9 Eh, tested). The SCF follows the reference's "
+                   "rules (CDIIS depth 10, convergence |dE| < 1e-10 and ||dP||_F < 1e-8) without damping and "
+                   "with pseudo-diagonalisation steps instead of an eigensolve in most iterations: iteration "
+                   "counts are this build's, not the reference's schedule"}
     # SAD (cold: atomic UHF densities recomputed; warm: cached per process) and the learned
     # guess (network trained on small water clusters and alkanes; "judged": the SAD start is
     # made too and the lower-energy start kept)
@@ -220,18 +233,29 @@ def main():
                 for name, inc, prec in (("fp64", False, "fp64"), ("fp64_incremental", True, "fp64"),
                                         ("mixed_incremental", True, "mixed"))
                 for cold in (True, False)]
-    variants += [("fp64_learned", False, "fp64", "LEARNED", False, False),
+    variants += [("fp64_learned", False, "fp64", "LEARNED", "auto", False),
                  ("fp64_learned_judged", False, "fp64", "LEARNED", True, False),
-                 ("mixed_incremental_learned", True, "mixed", "LEARNED", False, False),
+                 ("mixed_incremental_learned", True, "mixed", "LEARNED", "auto", False),
                  ("mixed_incremental_learned_judged", True, "mixed", "LEARNED", True, False)]
     for key, inc, prec, guess, judge, cold in variants:
-        r, wall = run_scf(inc, cold, prec, guess, judge)
-        scf[key] = {"energy": r.energy, "dE_vs_fp64": r.energy - E_ref, "iterations": r.iterations,
+        prof = key in ("fp64_incremental", "mixed_incremental")
+        r, wall = run_scf(inc, cold, prec, guess, judge, profile=prof)
+        t6 = time_to_energy(r, E_ref, 1e-6)
+        k6 = next(i for i, t in enumerate(r.times) if t >= t6)
+        scf[key] = {"energy": r.energy, "dE_vs_fp64": r.energy - E_ref,
+                    "dE_vs_oracle": (r.energy - E_oracle) if E_oracle is not None else None,
+                    "iterations": r.iterations, "iterations_to_1e-6": k6 + 1,
                     "wall_s": wall, "setup_s": r.setup_s, "setup_parts": r.setup_parts,
                     "guess_s": r.guess_s, "guess_used": str(r.guess_info.get("used")),
+                    "guess_judged": r.guess_info.get("judged"),
+                    "iterations_s_to_1e-6": t6 - r.guess_s,
                     "first_iteration_error": r.energies[0] - E_ref,
-                    "time_to_1e-6_s": r.setup_s + time_to_energy(r, E_ref, 1e-6),
+                    "time_to_1e-6_s": r.setup_s + t6,
                     "gd_ms_mean": 1e3 * float(np.mean(r.gd_times))}
+        if prof:
+            scf[key]["per_iteration"] = [{"build": ph["build"], "gd_ms": 1e3 * ph["gd"],
+                                          "quartets_fp64": ph["quartets_fp64"], "quartets_fp32": ph["quartets_fp32"]}
+                                         for ph in r.phase_times]
 
     # ---- batched uneven molecules: 128 alkanes C_nH_{2n+2}, n in [1, 12], and 128 (H2O)_m,
     # m in [1, 8], 6-31G*, one batched SCF; the same molecules one by one for comparison
@@ -249,9 +273,12 @@ def main():
             torch.cuda.synchronize()
             wall4 = time.perf_counter() - t0
             bs4.close()
+            # completion_s: when each molecule converged, counted from the batch's start (its latency
+            # in the batch); amortized: the batch wall time per molecule
             batch[key] = {"molecules_per_s": len(mols4) / wall4, "wall_s": wall4, "setup_s": rb.setup_s,
-                          "per_molecule_s_p50": float(np.nanmedian(rb.seconds)),
-                          "per_molecule_s_p99": float(np.nanpercentile(rb.seconds, 99)),
+                          "amortized_s_per_molecule": wall4 / len(mols4),
+                          "completion_s_p50": float(np.nanmedian(rb.seconds)),
+                          "completion_s_p99": float(np.nanpercentile(rb.seconds, 99)),
                           "iterations_max": int(rb.iterations.max()), "converged": int(rb.converged.sum())}
             if key == "fp64":
                 E4 = rb.energies
@@ -266,8 +293,8 @@ def main():
             seq_t.append(time.perf_counter() - t0)
             dE4 = max(dE4, abs(rs4.energy - E4[k * len(mols4) // nseq]))
         batch["one_by_one"] = {"molecules_per_s": nseq / sum(seq_t), "sample": f"{nseq} of the molecules, evenly spaced",
-                               "per_molecule_s_p50": float(np.median(seq_t)),
-                               "per_molecule_s_p99": float(np.percentile(seq_t, 99)),
+                               "scf_s_p50": float(np.median(seq_t)),
+                               "scf_s_p99": float(np.percentile(seq_t, 99)),
                                "max_abs_dE_vs_batch": dE4}
 
     if args.calibrate_copy:

@@ -24,7 +24,7 @@ CQ_PREC_MIXED = 1
 EXPORTED = [
     "cq_ctx_create", "cq_ctx_create_batch", "cq_ctx_destroy", "cq_last_error", "cq_set_schwarz_threshold",
     "cq_get_schwarz_threshold", "cq_set_precision_mode", "cq_get_dims", "cq_get_schwarz",
-    "cq_contract_direct", "cq_contract_direct_device", "cq_contract_direct_device_noalloc", "cq_get_quartet_counts",
+    "cq_contract_direct", "cq_contract_direct_device", "cq_contract_direct_device_noalloc", "cq_get_quartet_counts", "cq_get_precision_counts",
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
     "cq_get_kernel_times", "cq_get_screen_times", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_set_launch_streams", "cq_set_lds_digestion",
     "cq_solid_harmonics", "cq_set_class_digestion", "cq_get_class_digestion",
@@ -74,6 +74,7 @@ def load():
     L.cq_contract_direct_device_noalloc.argtypes = [vp, ctypes.c_int, P(vp), P(vp), ip, dp, ctypes.c_int,
                                                     ctypes.c_int, vp]
     L.cq_get_quartet_counts.argtypes = [vp, P(ctypes.c_longlong), ctypes.c_int, ip]
+    L.cq_get_precision_counts.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong)]
     L.cq_num_class_pairs.argtypes = []
     L.cq_class_pair_info.argtypes = [ctypes.c_int, ip, ip, ip, ip]
     L.cq_debug_quartets.argtypes = [vp, ctypes.c_int, ip, dp]

@@ -316,6 +316,12 @@ class AOIntegrals:
             out[name] = int(buf[k])
         return out
 
+    def precision_counts(self) -> Tuple[int, int]:
+        """Surviving quartets of the last contraction with FP64 and with FP32 integrals."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        check(_lib.load().cq_get_precision_counts(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
+        return a.value, b.value
+
     def quartets(self, shells4: np.ndarray) -> List[np.ndarray]:
         """Contracted quartets (s1 s2|s3 s4) in the shells' own basis (as libint returns them)."""
         s4 = np.ascontiguousarray(np.asarray(shells4, dtype=np.int32).reshape(-1, 4))

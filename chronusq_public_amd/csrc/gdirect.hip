@@ -139,21 +139,26 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 // Each wavefront takes one group of the class pair (groups item0 .. item0 + nitem - 1 of it: one
 // batch) and sweeps the other class's pairs of the group's molecule, sorted by Schwarz bound
-// inside each primitive-count group, so a sweep stops once no member can pass.  Each lane tests
-// its swept pair against every member with the reference's test
+// inside each primitive-count group, so a sweep stops once no member can pass; SCREEN_SW pairs
+// per lane and step, their loads issued together.  Each lane tests its swept pairs against
+// every member with the reference's test
 //   max(N(s1,s4), N(s2,s4), N(s3,s4), N(s1,s3), N(s2,s3), N(s1,s2)) * Q(s1,s2) * Q(s3,s4) >= tau
 // in the canonical order of the quartet (aointegrals_contract.h:111-131); the members it passes
 // form a bit mask.  Swept pairs with a non-empty mask join the union (LDS); a full union (or the
-// end of the sweep) is written out as one task region plus one segment per primitive block of
-// the members.  [tau, tau_hi) is the bound window of this launch (mixed precision).
+// end of the sweep) is written out as one task region into the group's own part of the task
+// list, with one segment per primitive block of the members (the group's reserved slots, then
+// the batch's overflow pool).  [tau, tau_hi) is the bound window of this launch (mixed precision).
+// (Measured alternatives: a whole 1024-thread workgroup per group, sharing the union, screens
+// (H2O)16/cc-pVDZ in 3.4 ms instead of 1.6 ms; one swept pair per lane and step, 1.7 ms.)
+#define SCREEN_SW 4
 __global__ void __launch_bounds__(WG) screen_kernel(
     const CPMeta* __restrict__ cps, int cp, int item0, int nitem, const GroupRec* __restrict__ groups,
     const int* __restrict__ uorder, const int2* __restrict__ s12, const double* __restrict__ pairQ,
     const int* __restrict__ grp, const int* __restrict__ cls_grp_off, const int* __restrict__ pair_mol,
     const int* __restrict__ cmg, int nmol, const double* __restrict__ Nmax, const double* __restrict__ Npair,
     const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, double tau_hi,
-    const unsigned char* __restrict__ pair_nblk, uint2* __restrict__ tasks, unsigned long long* __restrict__ counts,
-    unsigned long long* __restrict__ qcounts, Segment* __restrict__ segs, unsigned* __restrict__ nseg,
+    const unsigned char* __restrict__ pair_nblk, uint2* __restrict__ tasks,
+    unsigned long long* __restrict__ qcounts, Segment* __restrict__ segs, unsigned* __restrict__ ovf, int nres,
     int count_quartets) {
   __shared__ int ucode[WG / 64][UNION_MAX];
   __shared__ unsigned short umask[WG / 64][UNION_MAX];
@@ -195,7 +200,11 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   // the uniform pair's rows (cache-local for the whole wavefront)
   const bool symN = nglob_bits[1 + MAX_MATS] == 0ull;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  uint2* const tout = tasks + m.task_off;
+  // the group's own part of the batch's task list (no atomics) and its segment slots
+  uint2* const tout = tasks + m.task_off + gr.task_base;
+  Segment* const sout = segs + m.seg_off + gr.seg_base;
+  unsigned tcur = 0;
+  int scur = 0;
   int U = 0;
   unsigned long long nq = 0;
   auto flush = [&]() {
@@ -221,16 +230,18 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     wave_sync_lds();
     const unsigned hdr = 1u + (unsigned)n;
     const unsigned total = hdr + tot;
-    unsigned long long base = 0;
-    unsigned si = 0;
-    if (lane == 0) {
-      base = atomicAdd(&counts[cp], (unsigned long long)total);
-      si = atomicAdd(&nseg[cp], (unsigned)ublk);
+    // the first task region takes the group's reserved segment slots, later ones slots of the
+    // batch's overflow pool (after the reserved slots)
+    int sbase = 0;
+    if (scur > 0) {
+      unsigned o = 0;
+      if (lane == 0) o = atomicAdd(ovf, (unsigned)ublk);
+      sbase = nres + (int)__shfl(o, 0) - gr.seg_base;
     }
-    base = __shfl(base, 0);
-    si = __shfl(si, 0);
-    if (lane < ublk) segs[m.seg_off + si + lane] = Segment{(int)base, (int)total, g, ublk > 1 ? lane + 1 : 0};
-    uint2* out = tout + base;
+    if (lane < ublk) sout[sbase + lane] = Segment{gr.task_base + (int)tcur, (int)total, g, ublk > 1 ? lane + 1 : 0};
+    uint2* out = tout + tcur;
+    tcur += total;
+    scur += ublk;
     if (lane == 0) out[0] = make_uint2((unsigned)n, (unsigned)U);
     if (lane < n) {
       unsigned off = hdr;
@@ -262,17 +273,26 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   for (int gq = gm_lo; gq < gm_hi; ++gq) {
     const int grp_end = (gq + 1 < g_hi) ? grp[gq + 1] : nsw;
     if (grp_end <= i_begin) continue;
-    for (int i0 = max(grp[gq], i_begin); i0 < grp_end; i0 += 64) {
+    for (int i0 = max(grp[gq], i_begin); i0 < grp_end; i0 += 64 * SCREEN_SW) {
       const double Qi0 = pairQ[sw_start + i0];
       if (nglob * Qkmax * Qi0 * (1.0 + 1e-12) < tau) break;
-      const int i = i0 + lane;
-      unsigned mask = 0u;
-      if (i < grp_end) {
-        const int P = sw_start + i;
-        const double Qp = pairQ[P];
-        const int2 sp = s12[P];
-        const double Np = symN ? Npair[P] : 0.0;
-        const long canonP = (long)sp.x * (sp.x + 1) / 2 + sp.y;
+      double Qp[SCREEN_SW], Np[SCREEN_SW];
+      int2 sp[SCREEN_SW];
+#pragma unroll
+      for (int r = 0; r < SCREEN_SW; ++r) {
+        const int i = i0 + r * 64 + lane;
+        const int P = sw_start + (i < grp_end ? i : i0);
+        Qp[r] = pairQ[P];
+        sp[r] = s12[P];
+        Np[r] = symN ? Npair[P] : 0.0;
+      }
+      unsigned mask[SCREEN_SW];
+#pragma unroll
+      for (int r = 0; r < SCREEN_SW; ++r) {
+        const int i = i0 + r * 64 + lane;
+        mask[r] = 0u;
+        if (i >= grp_end) continue;
+        const long canonP = (long)sp[r].x * (sp[r].x + 1) / 2 + sp[r].y;
         for (int mm = 0; mm < n; ++mm) {
           const double Qk = mQ[wid][mm];
           if (nglob * Qk * Qi0 * (1.0 + 1e-12) < tau) continue;   // no swept pair from here passes mm
@@ -280,16 +300,16 @@ __global__ void __launch_bounds__(WG) screen_kernel(
           const int kx = ms1[wid][mm], ky = ms2[wid][mm];
           const long canonK = (long)kx * (kx + 1) / 2 + ky;
           const bool pfirst = canonP >= canonK;
-          const double Q12 = pfirst ? Qp : Qk, Q34 = pfirst ? Qk : Qp;
+          const double Q12 = pfirst ? Qp[r] : Qk, Q34 = pfirst ? Qk : Qp[r];
           double mx;
           if (symN) {
             const double* rx = Nmax + (long)kx * nshell;
             const double* ry = Nmax + (long)ky * nshell;
-            mx = fmax(fmax(Np, mN[wid][mm]), fmax(fmax(rx[sp.x], rx[sp.y]), fmax(ry[sp.x], ry[sp.y])));
+            mx = fmax(fmax(Np[r], mN[wid][mm]), fmax(fmax(rx[sp[r].x], rx[sp[r].y]), fmax(ry[sp[r].x], ry[sp[r].y])));
           } else {
             int s1, s2, s3, s4;
-            if (pfirst) { s1 = sp.x; s2 = sp.y; s3 = kx; s4 = ky; }
-            else        { s1 = kx; s2 = ky; s3 = sp.x; s4 = sp.y; }
+            if (pfirst) { s1 = sp[r].x; s2 = sp[r].y; s3 = kx; s4 = ky; }
+            else        { s1 = kx; s2 = ky; s3 = sp[r].x; s4 = sp[r].y; }
             const double n14 = Nmax[s1 * nshell + s4], n24 = Nmax[s2 * nshell + s4];
             const double n34 = Nmax[s3 * nshell + s4], n13 = Nmax[s1 * nshell + s3];
             const double n23 = Nmax[s2 * nshell + s3], n12 = Nmax[s1 * nshell + s2];
@@ -297,22 +317,27 @@ __global__ void __launch_bounds__(WG) screen_kernel(
           }
           // the reference's test, and the upper end of this launch's bound window
           const double bound = __dmul_rn(__dmul_rn(mx, Q12), Q34);
-          if (!(bound < tau) && bound < tau_hi) mask |= 1u << mm;
+          if (!(bound < tau) && bound < tau_hi) mask[r] |= 1u << mm;
         }
       }
-      nq += (unsigned long long)__popc(mask);
-      const bool has = mask != 0u;
-      const unsigned long long bal = __ballot(has);
-      if (has) {
-        const int slot = U + __popcll(bal & lt);
-        ucode[wid][slot] = sw_start + i;
-        umask[wid][slot] = (unsigned short)mask;
+#pragma unroll
+      for (int r = 0; r < SCREEN_SW; ++r) {
+        nq += (unsigned long long)__popc(mask[r]);
+        const bool has = mask[r] != 0u;
+        const unsigned long long bal = __ballot(has);
+        if (has) {
+          const int slot = U + __popcll(bal & lt);
+          ucode[wid][slot] = sw_start + i0 + r * 64 + lane;
+          umask[wid][slot] = (unsigned short)mask[r];
+        }
+        U += __popcll(bal);
+        if (U > m.ub - 64) flush();
       }
-      U += __popcll(bal);
-      if (U > m.ub - 64) flush();
     }
   }
   if (U > 0) flush();
+  // a group without survivors leaves its reserved slots empty
+  if (scur == 0 && lane < ublk) sout[lane] = Segment{0, 0, g, 0};
   // (a list screened again for a later slot or batch is not counted twice)
   nq = wave_sum(nq);
   if (lane == 0 && nq > 0 && count_quartets) atomicAdd(&qcounts[cp], nq);
@@ -550,16 +575,17 @@ __global__ void schwarz_kernel(const double* __restrict__ blocks, const int* __r
 // acc[1] += K_ab, acc[2] += sequential Cartesian->spherical transform cost, acc[3] += output
 // integrals in the shells' basis.  One wavefront per segment.
 __global__ void work_stats_kernel(const uint2* __restrict__ tasks, const Segment* __restrict__ segs,
-                                  const unsigned* __restrict__ nseg_ptr, const GroupRec* __restrict__ groups,
+                                  int nres, const unsigned* __restrict__ ovf, const GroupRec* __restrict__ groups,
                                   const int* __restrict__ uorder, int flip, const PairInfo* __restrict__ pairs,
                                   const ShellMeta* __restrict__ sh, double* __restrict__ acc) {
-  const long nseg = (long)*nseg_ptr;
+  const long nseg = (long)nres + *ovf;
   const int lane = threadIdx.x & 63;
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const long nwaves = (long)gridDim.x * blockDim.x / 64;
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (long si = wave; si < nseg; si += nwaves) {
     const Segment sg = segs[si];
+    if (sg.count == 0) continue;
     const GroupRec gr = groups[sg.ket];
     const uint2* reg = tasks + sg.off;
     for (int m = 0; m < gr.n; ++m) {
@@ -767,6 +793,7 @@ struct cq_ctx {
   bool plan_ok = false;   // task and segment buffers are allocated for the current plan
   double* scratch = nullptr;   // this device's address of the no-allocation workspace
   std::vector<std::vector<int>> batch_start[2];
+  std::vector<std::vector<int>> batch_slots[2];   // reserved segment slots of each batch (one set per group)
   std::vector<unsigned char> nblk_h, nublk_h;
   std::vector<int> pair_mol_h;
   std::vector<unsigned long long> last_counts;
@@ -815,13 +842,13 @@ struct cq_ctx {
   DevBuf<CPMeta> d_cps[2];
   DevBuf<uint2> d_tasks;
   DevBuf<Segment> d_segs;
-  DevBuf<unsigned> d_nseg, d_work_q;
+  DevBuf<unsigned> d_work_q, d_ovf;
   int lds_mode = 2;   // 0 HBM atomics, 1 ket-keyed LDS segments, 2 also bra-keyed segments
   // per class pair: the digestion variant (0..2 as lds_mode, capped by it), -1 = lds_mode
   std::vector<int> class_variant = std::vector<int>(CQ_NUM_CLASS_PAIRS, -1);
   // per class pair: the auxiliary stream its kernels go to (-1: round robin in launch order)
   std::vector<int> class_stream = std::vector<int>(CQ_NUM_CLASS_PAIRS, -1);
-  DevBuf<unsigned long long> d_counts, d_qcounts, d_nglob;
+  DevBuf<unsigned long long> d_qcounts, d_nglob;
   DevBuf<unsigned char> d_pair_nblk, d_pair_nublk;
   std::string err;
 
@@ -829,7 +856,7 @@ struct cq_ctx {
     d_shells.release(); d_cart2shell.release(); d_sph2shell.release(); d_tco_off.release();
     d_uorder.release(); d_groups[0].release(); d_groups[1].release(); d_flags.release(); d_tco.release(); d_boys.release(); d_pairQ.release();
     d_Nmax.release(); d_Npair.release(); d_work.release(); d_mats.release(); d_pairs.release(); d_pairs_1e.release(); d_prims.release();
-    d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_nseg.release(); d_work_q.release(); d_counts.release(); d_qcounts.release(); d_nglob.release(); d_pair_nblk.release(); d_pair_nublk.release();
+    d_cps[0].release(); d_cps[1].release(); d_tasks.release(); d_segs.release(); d_work_q.release(); d_ovf.release(); d_qcounts.release(); d_nglob.release(); d_pair_nblk.release(); d_pair_nublk.release();
     d_prim_off.release(); d_alpha.release(); d_coef.release(); d_origin.release(); d_onee.release();
     for (auto e : event_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < kAux; ++k) {
@@ -925,6 +952,7 @@ void plan_tasks(cq_ctx* c) {
     c->cps[t].clear();
     c->groups_h[t].clear();
     c->batch_start[t].assign(kNumClassPairs, std::vector<int>());
+    c->batch_slots[t].assign(kNumClassPairs, std::vector<int>());
   }
   const long long budget = c->task_budget;
   long long off = 0, seg_total[2] = {0, 0};
@@ -975,7 +1003,7 @@ void plan_tasks(cq_ctx* c) {
           if (pair_mol[P] != pair_mol[P0] || nublk[P] != nublk[P0] || c->pairs[P].a != c->pairs[P0].a) break;
           ++y;
         }
-        c->groups_h[t].push_back(GroupRec{x, y - x, nublk[P0], 0});
+        c->groups_h[t].push_back(GroupRec{x, y - x, nublk[P0], 0, 0, 0, 0, 0});
         x = y;
       }
       mc.ngrp = (int)c->groups_h[t].size() - mc.grp_off;
@@ -991,17 +1019,23 @@ void plan_tasks(cq_ctx* c) {
         }
       }
       std::vector<int>& bs = c->batch_start[t][k];
+      std::vector<int>& bslots = c->batch_slots[t][k];
       bs.assign(1, 0);
-      long long cap = 0, cur = 0, seg_cap = 0, seg_cur = 0;
+      bslots.clear();
+      // segments of a batch: each group's reserved slots (its first task region), then the
+      // overflow pool for the groups' further task regions
+      long long cap = 0, cur = 0, seg_cap = 0, seg_cur = 0, seg_ovf = 0;
       auto close = [&](int gi) {
         bs.push_back(gi);
+        bslots.push_back((int)seg_cur);
         cap = std::max(cap, cur);
-        seg_cap = std::max(seg_cap, seg_cur);
+        seg_cap = std::max(seg_cap, seg_cur + seg_ovf);
         cur = 0;
         seg_cur = 0;
+        seg_ovf = 0;
       };
       for (int gi = 0; gi < mc.ngrp; ++gi) {
-        const GroupRec& gr = c->groups_h[t][mc.grp_off + gi];
+        GroupRec& gr = c->groups_h[t][mc.grp_off + gi];
         const int mol = pair_mol[c->uorder_h[gr.first]];
         long long S, Bk;
         if (m.same) {
@@ -1016,12 +1050,17 @@ void plan_tasks(cq_ctx* c) {
         const long long nflush = S / (ub - 63) + 1;
         const long long w = (long long)gr.n * Bk + nflush * (1 + gr.n);
         if (gi > bs.back() && cur + w > budget) close(gi);
+        // task offsets within a list are 32-bit (Segment::off, GroupRec::task_base)
+        if (cur + w > 0x7fffffffLL || seg_cur + seg_ovf + nflush * gr.ublk > 0x7fffffffLL)
+          throw std::invalid_argument("one batch's task list exceeds 2^31 entries");
+        gr.task_base = (int)cur;
+        gr.seg_base = (int)seg_cur;
+        gr.nslot = gr.ublk;
         cur += w;
-        seg_cur += nflush * gr.ublk;
+        seg_cur += gr.ublk;
+        seg_ovf += (nflush - 1) * gr.ublk;
       }
       close(mc.ngrp);
-      // task offsets within a list are 32-bit (Segment::off)
-      if (cap > 0x7fffffffLL) throw std::invalid_argument("one group's task list exceeds 2^31 entries");
       mc.task_cap = cap;
       mc.seg_cap = seg_cap;
       mc.seg_off = seg_total[t];
@@ -1351,10 +1390,10 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_uorder.upload(c->uorder_h, c->stream);
   plan_tasks(c);
   c->scratch = scratch_address();
-  c->d_nseg.ensure(kNumClassPairs);
-  c->d_work_q.ensure((size_t)kNumClassPairs * MAX_MATS);   // one work queue per class pair and slot pass
-  c->d_counts.ensure(kNumClassPairs);
-  c->d_qcounts.ensure(kNumClassPairs);
+  // WORK_QUEUES work queues per class pair and slot pass; one overflow-slot counter per class pair
+  c->d_work_q.ensure((size_t)kNumClassPairs * MAX_MATS * WORK_QUEUES);
+  c->d_ovf.ensure(kNumClassPairs);
+  c->d_qcounts.ensure(2 * kNumClassPairs);   // surviving quartets: FP64 window, then FP32 window
   c->d_nglob.ensure(2 + MAX_MATS);   // largest |X| of the call, of each matrix, norm asymmetry flag
   c->d_Nmax.ensure((size_t)nshell * nshell);
   c->d_Npair.ensure(std::max(1, npair));
@@ -1437,10 +1476,9 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   };
   zero(c->d_nglob.p, (2 + MAX_MATS) * sizeof(unsigned long long));
   zero(flags, std::max(1, nslot) * sizeof(int));
-  zero(c->d_counts.p, kNumClassPairs * sizeof(unsigned long long));
-  zero(c->d_qcounts.p, kNumClassPairs * sizeof(unsigned long long));
-  zero(c->d_nseg.p, kNumClassPairs * sizeof(unsigned));
-  zero(c->d_work_q.p, (size_t)kNumClassPairs * MAX_MATS * sizeof(unsigned));
+  zero(c->d_qcounts.p, 2 * kNumClassPairs * sizeof(unsigned long long));
+  zero(c->d_work_q.p, (size_t)kNumClassPairs * MAX_MATS * WORK_QUEUES * sizeof(unsigned));
+  zero(c->d_ovf.p, kNumClassPairs * sizeof(unsigned));
   // the whole workspace (the Cartesian densities in it are rewritten below)
   zero(work, (size_t)std::max(1, nslot) * 5 * nc2 * sizeof(double));
   hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(WG), 0, s, zl);
@@ -1489,9 +1527,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
         zr.n[zr.count] = (long)(bytes / 4);
         ++zr.count;
       };
-      zero2(c->d_counts.p, kNumClassPairs * sizeof(unsigned long long));
-      zero2(c->d_nseg.p, kNumClassPairs * sizeof(unsigned));
-      zero2(c->d_work_q.p, (size_t)kNumClassPairs * MAX_MATS * sizeof(unsigned));
+      zero2(c->d_work_q.p, (size_t)kNumClassPairs * MAX_MATS * WORK_QUEUES * sizeof(unsigned));
+      zero2(c->d_ovf.p, kNumClassPairs * sizeof(unsigned));
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(WG), 0, s, zr);
       HIPCHK(hipGetLastError());
     }
@@ -1532,7 +1569,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
         throw std::runtime_error(std::string("class pair ") + ck.name + ": the planned tile does not fit the device's LDS");
       GroupKernel kern = ck.grp[fp32 ? 1 : 0][kv];
       const int wgs = kv == 2 ? WG : ck.seg_wg;
-      unsigned* wq = c->d_work_q.p + (size_t)s0 * kNumClassPairs + k;
+      unsigned* wq = c->d_work_q.p + ((size_t)s0 * kNumClassPairs + k) * WORK_QUEUES;
       // batches of groups, in order on this class pair's stream: a class pair in one batch
       // screens once and keeps its lists for every slot pass; a batched one clears its counters
       // and screens again before each batch
@@ -1540,15 +1577,14 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const int nbatch = (int)bs.size() - 1;
       for (int b = 0; b < nbatch; ++b) {
         const int g0 = bs[b], ng = bs[b + 1] - bs[b];
+        const int nslots = c->batch_slots[tb][k][b];
         if (nbatch > 1) {
           ZeroList zb{};
-          zb.p[0] = (unsigned*)(c->d_counts.p + k);
-          zb.n[0] = 2;
-          zb.p[1] = c->d_nseg.p + k;
+          zb.p[0] = wq;
+          zb.n[0] = WORK_QUEUES;
+          zb.p[1] = c->d_ovf.p + k;
           zb.n[1] = 1;
-          zb.p[2] = wq;
-          zb.n[2] = 1;
-          zb.count = 3;
+          zb.count = 2;
           hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, sk, zb);
           HIPCHK(hipGetLastError());
         }
@@ -1563,8 +1599,9 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
                              c->d_groups[tb].p, c->d_uorder.p, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
                              c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p, c->nmol, c->d_Nmax.p, c->d_Npair.p,
                              c->d_nglob.p, ns,
-                             lo, hi, c->d_pair_nblk.p, c->d_tasks.p, c->d_counts.p, c->d_qcounts.p, c->d_segs.p,
-                             c->d_nseg.p, s0 == 0 ? 1 : 0);
+                             lo, hi, c->d_pair_nblk.p, c->d_tasks.p, c->d_qcounts.p + (fp32 ? kNumClassPairs : 0),
+                             c->d_segs.p, c->d_ovf.p + k,
+                             nslots, s0 == 0 ? 1 : 0);
           HIPCHK(hipGetLastError());
           if (profiling) {
             HIPCHK(hipEventRecord(q1, sk));
@@ -1578,9 +1615,11 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
           e1 = c->get_event();
           HIPCHK(hipEventRecord(e0, sk));
         }
+        // (the overflow regions are known on the device only: the grid covers the segment capacity)
         const int sgrid = (int)std::min<long long>((long long)persistent_grid() * WG / wgs, m.seg_cap * ck.nchunk);
         hipLaunchKernelGGL(kern, dim3(std::max(1, sgrid)), dim3(wgs), lds_bytes, sk, c->d_tasks.p + m.task_off,
-                           c->d_segs.p + m.seg_off, c->d_nseg.p + k, wq, c->d_pairs.p, c->d_prims.p, c->d_boys.p, jk,
+                           c->d_segs.p + m.seg_off, nslots, c->d_ovf.p + k, wq, c->d_pairs.p, c->d_prims.p,
+                           c->d_boys.p, jk,
                            c->d_groups[tb].p, c->d_uorder.p, m.mmax, m.ub);
         HIPCHK(hipGetLastError());
         if (profiling) {
@@ -1847,10 +1886,25 @@ int cq_contract_direct_device_noalloc(cq_ctx_t* c, int nmat, const double* const
 int cq_get_quartet_counts(cq_ctx_t* c, long long* counts, int max_counts, int* n_out) {
   return guarded(c, [&] {
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<unsigned long long> h(kNumClassPairs);
-    HIPCHK(hipMemcpy(h.data(), c->d_qcounts.p, kNumClassPairs * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    for (int k = 0; k < kNumClassPairs && k < max_counts; ++k) counts[k] = (long long)h[k];
+    std::vector<unsigned long long> h(2 * kNumClassPairs);
+    HIPCHK(hipMemcpy(h.data(), c->d_qcounts.p, 2 * kNumClassPairs * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kNumClassPairs && k < max_counts; ++k) counts[k] = (long long)(h[k] + h[kNumClassPairs + k]);
     if (n_out) *n_out = kNumClassPairs;
+  });
+}
+
+// Surviving quartets of the last contraction by arithmetic: FP64 integrals and (mixed precision
+// only) FP32 integrals, summed over class pairs.
+int cq_get_precision_counts(cq_ctx_t* c, long long* fp64, long long* fp32) {
+  return guarded(c, [&] {
+    if (!fp64 || !fp32) throw std::invalid_argument("null output");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<unsigned long long> h(2 * kNumClassPairs);
+    HIPCHK(hipMemcpy(h.data(), c->d_qcounts.p, 2 * kNumClassPairs * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    long long a = 0, b = 0;
+    for (int k = 0; k < kNumClassPairs; ++k) { a += (long long)h[k]; b += (long long)h[kNumClassPairs + k]; }
+    *fp64 = a;
+    *fp32 = b;
   });
 }
 
@@ -2112,7 +2166,8 @@ int cq_get_work_stats(cq_ctx_t* c, double* out, int max_n) {
       const CPMeta& m = c->cps[tb][k];
       if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
       hipLaunchKernelGGL(work_stats_kernel, dim3(persistent_grid()), dim3(WG), 0, c->stream,
-                         c->d_tasks.p + m.task_off, c->d_segs.p + m.seg_off, c->d_nseg.p + k, c->d_groups[tb].p,
+                         c->d_tasks.p + m.task_off, c->d_segs.p + m.seg_off, c->batch_slots[tb][k][0], c->d_ovf.p + k,
+                         c->d_groups[tb].p,
                          c->d_uorder.p, m.flip, c->d_pairs.p, c->d_shells.p, acc.p + 4 * k);
       HIPCHK(hipGetLastError());
     }

@@ -23,11 +23,19 @@
 // the work item once.
 #define MAX_GROUP 16
 struct GroupRec {
-  int first;   // into uorder
-  int n;       // members
-  int ublk;    // primitive blocks of the uniform side (equal for every member)
-  int pad;
+  int first;       // into uorder
+  int n;           // members
+  int ublk;        // primitive blocks of the uniform side (equal for every member)
+  int nslot;       // segment slots reserved for the group's first task region (ublk)
+  int task_base;   // the group's task entries, from the start of its batch's task list
+  int seg_base;    // the group's segment slots, from the start of its batch's segment list
+  int pad0, pad1;
 };
+// Every group owns a fixed part of its batch's task list, sized by the plan's worst case, and
+// ublk reserved segment slots for its first task region, so the screening places its output
+// without atomics; further task regions of a group (a union over the bound) take slots of the
+// batch's overflow pool (one atomic each).  A group without survivors writes its reserved
+// slots with count 0; the kernels skip them.
 
 // ---------------------------------------------------------------------------------------
 // ERI kernel wrappers, instantiated once per class pair by the generated units
@@ -84,10 +92,18 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
   q.loff = B.loff;   // the uniform bra's molecule
 }
 
-// Take the next work item from the queue (whole workgroup).
+// Take the next work item (whole workgroup).  Up to eight queues, one per XCD (workgroups are
+// dispatched round robin over the XCDs): queue x hands out items x, x + nq, x + 2 nq, ... to the
+// workgroups with blockIdx % nq == x, so the counters do not contend across the chip and every
+// queue has a consumer (nq <= gridDim).
+#define WORK_QUEUES 8
 __device__ __forceinline__ long next_item(unsigned* __restrict__ work, int* s_item) {
   __syncthreads();
-  if (threadIdx.x == 0) *s_item = (int)atomicAdd(work, 1u);
+  if (threadIdx.x == 0) {
+    const unsigned nq = gridDim.x < WORK_QUEUES ? gridDim.x : WORK_QUEUES;
+    const unsigned x = blockIdx.x % nq;
+    *s_item = (int)(x + nq * atomicAdd(&work[x], 1u));
+  }
   __syncthreads();
   return (long)__builtin_amdgcn_readfirstlane(*s_item);
 }
@@ -114,7 +130,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
   template <int WGS, int DIG, typename R>                                                     \
   __global__ void __launch_bounds__(WGS) eri_grp_##NAME(                                       \
       const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                      \
-      const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                     \
+      int nres, const unsigned* __restrict__ ovf, unsigned* __restrict__ work,                \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                 \
       const double* __restrict__ boys, JKArgs jk, const GroupRec* __restrict__ groups,        \
       const int* __restrict__ uorder, int mmax, int ub) {                                     \
@@ -124,7 +140,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
     __shared__ int s_item;                                                                    \
     __shared__ int s_dcol[MAX_GROUP], s_moff[MAX_GROUP];                                      \
     extern __shared__ double dyn_lds[];                                                       \
-    const long nitems = (long)(*nseg_ptr) * NCH;                                              \
+    const long nitems = ((long)nres + *ovf) * NCH;                                            \
     if (nitems == 0) return;                                                                  \
     stage_boys(boys, LTOT, boys_lds);                                                         \
     const int N = jk.ncart, LN = jk.lds_n;                                                    \
@@ -139,6 +155,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
       const long item = next_item(work, &s_item);                                             \
       if (item >= nitems) break;                                                              \
       const Segment sg = segs[item / NCH];                                                    \
+      if (sg.count == 0) continue;   /* a slot its group did not fill */                      \
       const int chunk = (int)(item % NCH);                                                    \
       const GroupRec gr = groups[sg.ket];                                                     \
       const int n = gr.n;                                                                     \
@@ -239,7 +256,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
   template <typename R>                                                                         \
   __global__ void __launch_bounds__(WG) eri_grpb_##NAME(                                        \
       const uint2* __restrict__ tasks, const Segment* __restrict__ segs,                        \
-      const unsigned* __restrict__ nseg_ptr, unsigned* __restrict__ work,                       \
+      int nres, const unsigned* __restrict__ ovf, unsigned* __restrict__ work,                  \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                   \
       const double* __restrict__ boys, JKArgs jk, const GroupRec* __restrict__ groups,          \
       const int* __restrict__ uorder, int mmax, int ub) {                                       \
@@ -249,7 +266,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
     __shared__ int s_item;                                                                      \
     __shared__ int s_brow[MAX_GROUP], s_moff[MAX_GROUP];                                        \
     extern __shared__ double dyn_lds[];                                                         \
-    const long nitems = (long)(*nseg_ptr) * NCH;                                                \
+    const long nitems = ((long)nres + *ovf) * NCH;                                              \
     if (nitems == 0) return;                                                                    \
     stage_boys(boys, LTOT, boys_lds);                                                           \
     const int N = jk.ncart, LN = jk.lds_n;                                                      \
@@ -264,6 +281,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
       const long item = next_item(work, &s_item);                                               \
       if (item >= nitems) break;                                                                \
       const Segment sg = segs[item / NCH];                                                      \
+      if (sg.count == 0) continue;   /* a slot its group did not fill */                        \
       const int chunk = (int)(item % NCH);                                                      \
       const GroupRec gr = groups[sg.ket];                                                       \
       const int n = gr.n;                                                                       \
@@ -333,7 +351,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
 
 typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int,
                             const int*);
-typedef void (*GroupKernel)(const uint2*, const Segment*, const unsigned*, unsigned*, const PairInfo*,
+typedef void (*GroupKernel)(const uint2*, const Segment*, int, const unsigned*, unsigned*, const PairInfo*,
                             const PrimPair*, const double*, JKArgs, const GroupRec*, const int*, int, int);
 
 struct ClassKernels {

@@ -97,7 +97,7 @@ class SingleSlater:
                  incremental: bool = False, rebuild_every: int = 8, sad_cache: bool = True,
                  precision: str = "fp64", mixed_threshold: float = 1e-5,
                  pseudo_diag: bool = True, pdiag_switch: float = 1.0, pdiag_first: bool = True,
-                 guess_model: Optional[str] = None, judge_guess: bool = True,
+                 guess_model: Optional[str] = None, judge_guess="auto", judge_idem_tol: float = 0.15,
                  restart_file: Optional[str] = None):
         import torch
         self.torch = torch
@@ -114,7 +114,13 @@ class SingleSlater:
         self.sad_cache = sad_cache
         # learned guess: network file (default: the one trained for this basis) and whether
         # the SAD density competes with it on energy
-        self.guess_model, self.judge_guess = guess_model, judge_guess
+        # judge_guess: True always makes the SAD start too and keeps the lower-energy one; "auto"
+        # does so only when the network's prediction is not nearly idempotent (relative error
+        # ||P S P / 2 - P||_F / ||P||_F above judge_idem_tol, or a wrong electron count): a
+        # prediction outside what the network has learned; False never
+        if judge_guess not in (True, False, "auto"):
+            raise ValueError("judge_guess must be True, False or 'auto'")
+        self.guess_model, self.judge_guess, self.judge_idem_tol = guess_model, judge_guess, judge_idem_tol
         self.guess_info: Dict[str, object] = {}
         # restart file in the reference's layout (restart.py): the READ guess reads its
         # densities; SCF3 writes the converged ones into it
@@ -310,12 +316,13 @@ class SingleSlater:
 
     def learned_guess(self):
         """Atomic densities plus network-predicted pair blocks and block corrections
-        (learned_guess.py), then one Fock build and diagonalisation as for SAD.  With
-        judge_guess the SAD starting density is made as well, and of the two idempotent
-        starting densities the one with the lower (variational) energy E[P0] = Tr(P0 H) +
-        1/4 Tr(P0 PT[P0]) + E_nuc is kept; its PT[P0] becomes the SCF's first Fock build, so
-        judging costs two extra G[D] builds.  Molecules or bases the network does not cover
-        take the SAD guess.  No reference counterpart."""
+        (learned_guess.py), then one Fock build and diagonalisation as for SAD.  When judged
+        (judge_guess True, or "auto" with a prediction far from idempotent) the SAD starting
+        density is made as well, and of the two idempotent starting densities the one with the
+        lower (variational) energy E[P0] = Tr(P0 H) + 1/4 Tr(P0 PT[P0]) + E_nuc is kept; its
+        PT[P0] becomes the SCF's first Fock build, so judging costs an eigensolve and two extra
+        G[D] builds.  Molecules or bases the network does not cover take the SAD guess.  No
+        reference counterpart."""
         from . import learned_guess as LG
         if self.mol.natom <= 1:
             return self.sad_guess()
@@ -327,9 +334,16 @@ class SingleSlater:
             self.guess_info["fallback"] = str(e)
             return out
         uhf = self.ref != "RHF"
+        # the prediction's distance from an RHF density of the right electron count
+        torch = self.torch
+        PS = PL @ self.S
+        idem = float(torch.linalg.norm(PS @ PL * 0.5 - PL) / torch.linalg.norm(PL))
+        nel = float(torch.trace(PS))
+        confident = idem <= self.judge_idem_tol and abs(nel - self.mol.nelectron) <= 0.05 * self.mol.nelectron
         P0_L = self._guess_from_fock(self.formPT(PL)[0])
-        self.guess_info = {"used": "LEARNED"}
-        if not self.judge_guess:
+        self.guess_info = {"used": "LEARNED", "idempotency_error": idem, "electrons": nel,
+                           "judged": bool(self.judge_guess is True or (self.judge_guess == "auto" and not confident))}
+        if not self.guess_info["judged"]:
             return P0_L
         C_L = self._guess_C
         P0_S = self._guess_from_fock(self.formPT(self._atomic_densities())[0])
@@ -403,6 +417,7 @@ class SingleSlater:
             # hold the 1e-10 / 1e-8 criteria back near convergence
             if it == 0 and first_PT is not None and first_PT[0] is P:
                 PT, PTz = first_PT[1], first_PT[2]
+                kind = "guess"
             elif (self.incremental and not uhf and it % self.rebuild_every != 0
                     and abs(last_dE) > 1e-7):
                 if self.precision == "mixed":
@@ -411,8 +426,10 @@ class SingleSlater:
                 self.ao.setPrecision("fp64")
                 PT = PT_prev + dPT
                 PTz = None
+                kind = "incremental"
             else:
                 PT, PTz = self.formPT(P, Mz if uhf else None)
+                kind = "full"
             P_prev, PT_prev = P, PT
             torch.cuda.synchronize()
             res.gd_times.append(time.perf_counter() - tg)
@@ -420,7 +437,8 @@ class SingleSlater:
             if profile:
                 torch.cuda.synchronize()
                 t_ph = time.perf_counter()
-                ph = {"gd": t_ph - tg}
+                q64, q32 = self.ao.precision_counts() if kind != "guess" else (0, 0)
+                ph = {"gd": t_ph - tg, "quartets_fp64": q64, "quartets_fp32": q32, "build": kind}
             if uhf:
                 Fa = self.H + 0.5 * (PT + PTz)
                 Fb = self.H + 0.5 * (PT - PTz)

@@ -131,6 +131,9 @@ int cq_solid_harmonics(int l, double* out);
 
 /* Surviving quartets per class pair of the last contraction (synchronises). */
 int cq_get_quartet_counts(cq_ctx_t* ctx, long long* counts, int max_counts, int* n_out);
+/* Surviving quartets of the last contraction by arithmetic: FP64 integrals and FP32 integrals
+ * (the mixed-precision window; 0 in FP64 mode), summed over class pairs. */
+int cq_get_precision_counts(cq_ctx_t* ctx, long long* fp64, long long* fp32);
 int cq_num_class_pairs(void);
 int cq_class_pair_info(int k, int* la, int* lb, int* lc, int* ld);
 

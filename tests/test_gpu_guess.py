@@ -175,7 +175,7 @@ def test_learned_guess_fewer_iterations_same_energy(basis_name, seed):
     from chronusq_public_amd.singleslater import SingleSlater
     mol = M.water_cluster(16, seed) if seed == 0 else _held_out_water(16, seed)
     rs = SingleSlater(mol, basis_name, guess="SAD").SCF3()
-    rl = SingleSlater(mol, basis_name, guess="LEARNED").SCF3()
+    rl = SingleSlater(mol, basis_name, guess="LEARNED", judge_guess=True).SCF3()
     assert rl.guess_info["used"] == "LEARNED"
     assert rl.guess_info["E0_learned"] < rl.guess_info["E0_sad"]
     assert abs(rl.energy - rs.energy) < 1e-8
@@ -222,7 +222,24 @@ def test_judge_keeps_sad_over_an_untrained_network(tmp_path):
     mol = _held_out_water(4, 1)
     rl = SingleSlater(mol, "6-31g*", guess="LEARNED", guess_model=path).SCF3()
     rs = SingleSlater(mol, "6-31g*", guess="SAD").SCF3()
+    # the default ("auto") judges because the untrained prediction is far from idempotent
+    assert rl.guess_info["judged"] and rl.guess_info["idempotency_error"] > 0.15
     assert rl.guess_info["used"] == "SAD"
     assert rl.guess_info["E0_sad"] < rl.guess_info["E0_learned"]
     assert abs(rl.energy - rs.energy) < 1e-9
     assert abs(rl.iterations - rs.iterations) <= 1
+
+
+def test_auto_judge_trusts_a_confident_prediction():
+    """The trained network's prediction for a held-out cluster is nearly idempotent with the
+    right electron count: "auto" skips the SAD start (no extra eigensolve and G[D] builds) and
+    reaches the same energy; judge_guess=True still compares the two starts."""
+    from chronusq_public_amd.singleslater import SingleSlater
+    mol = _held_out_water(8, 2)
+    ra = SingleSlater(mol, "6-31g*", guess="LEARNED").SCF3()
+    rj = SingleSlater(mol, "6-31g*", guess="LEARNED", judge_guess=True).SCF3()
+    print(f"[guess] idempotency error {ra.guess_info['idempotency_error']:.3e}, "
+          f"electrons {ra.guess_info['electrons']:.4f}", flush=True)
+    assert not ra.guess_info["judged"] and ra.guess_info["used"] == "LEARNED"
+    assert rj.guess_info["judged"] and "E0_sad" in rj.guess_info
+    assert abs(ra.energy - rj.energy) < 1e-9
