@@ -171,6 +171,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-molecule SCF")
     ap.add_argument("--no-autotune", action="store_true", help="skip the per-class digestion autotuner")
+    ap.add_argument("--no-scf-variants", action="store_true",
+                    help="run only the reference FP64 SCF (its converged density feeds the G[D] steps), "
+                         "not the guess / incremental / mixed-precision variants")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run three 256 MiB device-to-device copies (a plain copy kernel of "
                          "known size, the scale for HBM byte counters)")
@@ -237,6 +240,8 @@ def main():
                  ("fp64_learned_judged", False, "fp64", "LEARNED", True, False),
                  ("mixed_incremental_learned", True, "mixed", "LEARNED", "auto", False),
                  ("mixed_incremental_learned_judged", True, "mixed", "LEARNED", True, False)]
+    if args.no_scf_variants:
+        variants = []
     for key, inc, prec, guess, judge, cold in variants:
         prof = key in ("fp64_incremental", "mixed_incremental")
         r, wall = run_scf(inc, cold, prec, guess, judge, profile=prof)
