@@ -350,9 +350,10 @@ def main():
     ms = (ctypes.c_double * ncp)()
     nl = (ctypes.c_longlong * ncp)()
     _lib.check(L.cq_get_kernel_times(ao._h, ms, nl, ncp), ao._h)
-    sms = (ctypes.c_double * ncp)()
-    snl = (ctypes.c_longlong * ncp)()
-    _lib.check(L.cq_get_screen_times(ao._h, sms, snl, ncp), ao._h)
+    sms = (ctypes.c_double * (ncp + 1))()      # entry ncp: the fused screening launch
+    snl = (ctypes.c_longlong * (ncp + 1))()
+    _lib.check(L.cq_get_screen_times(ao._h, sms, snl, ncp + 1), ao._h)
+    fused_screen_ms = sms[ncp] / snl[ncp] if snl[ncp] else 0.0
     _lib.check(L.cq_set_profiling(ao._h, 0), ao._h)
     _lib.check(L.cq_set_launch_streams(ao._h, 4), ao._h)
     counts = ao.quartet_counts()
@@ -401,7 +402,8 @@ def main():
                    "prim_quartets_per_s": prim_q / (ms_per_step * 1e-3),
                    "integrals_per_s": n_int / (ms_per_step * 1e-3),
                    "serial_kernels_ms": sum(r["kernel_ms"] for r in per_class),
-                   "serial_screen_ms": sum(r["screen_ms"] for r in per_class),
+                   "serial_screen_ms": sum(r["screen_ms"] for r in per_class) + fused_screen_ms,
+                   "fused_screen_ms": fused_screen_ms,
                    "all_kernels_tflops": total_flops / (sum(r["kernel_ms"] for r in per_class) * 1e-3) / 1e12,
                    "step_tflops": total_flops / (ms_per_step * 1e-3) / 1e12},
         "roofline": {"bound": "fp64-valu",

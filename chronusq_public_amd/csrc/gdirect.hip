@@ -151,14 +151,26 @@ __device__ __forceinline__ T wave_sum(T v) {
 // (Measured alternatives: a whole 1024-thread workgroup per group, sharing the union, screens
 // (H2O)16/cc-pVDZ in 3.4 ms instead of 1.6 ms; one swept pair per lane and step, 1.7 ms.)
 #define SCREEN_SW 4
+// The (class pair, batch of groups) entries one screening launch covers: entry e takes waves
+// cum[e] .. cum[e + 1] - 1, one per group item0[e] + (wave - cum[e]) of class pair cp[e], whose
+// plan is in table tb[e] (cps[tb], groups[tb]) and whose batch reserves nres[e] segment slots.
+// One launch may hold every class pair that screens once per call (all in one batch): the
+// waves of all of them share the device instead of 21 launches of a few hundred waves each.
+struct ScreenList {
+  const CPMeta* cps[2];
+  const GroupRec* groups[2];
+  int n;
+  int cp[CQ_NUM_CLASS_PAIRS], tb[CQ_NUM_CLASS_PAIRS], item0[CQ_NUM_CLASS_PAIRS], nres[CQ_NUM_CLASS_PAIRS];
+  int cum[CQ_NUM_CLASS_PAIRS + 1];
+};
+
 __global__ void __launch_bounds__(WG) screen_kernel(
-    const CPMeta* __restrict__ cps, int cp, int item0, int nitem, const GroupRec* __restrict__ groups,
-    const int* __restrict__ uorder, const int2* __restrict__ s12, const double* __restrict__ pairQ,
+    ScreenList sl, const int* __restrict__ uorder, const int2* __restrict__ s12, const double* __restrict__ pairQ,
     const int* __restrict__ grp, const int* __restrict__ cls_grp_off, const int* __restrict__ pair_mol,
     const int* __restrict__ cmg, int nmol, const double* __restrict__ Nmax, const double* __restrict__ Npair,
     const unsigned long long* __restrict__ nglob_bits, int nshell, double tau, double tau_hi,
     const unsigned char* __restrict__ pair_nblk, uint2* __restrict__ tasks,
-    unsigned long long* __restrict__ qcounts, Segment* __restrict__ segs, unsigned* __restrict__ ovf, int nres,
+    unsigned long long* __restrict__ qcounts, Segment* __restrict__ segs, unsigned* __restrict__ ovf_all,
     int count_quartets) {
   __shared__ int ucode[WG / 64][UNION_MAX];
   __shared__ unsigned short umask[WG / 64][UNION_MAX];
@@ -168,10 +180,15 @@ __global__ void __launch_bounds__(WG) screen_kernel(
   const int wid = threadIdx.x >> 6;
   const int wave = blockIdx.x * (WG / 64) + wid;
   const int lane = threadIdx.x & 63;
-  if (wave >= nitem) return;
-  const CPMeta m = cps[cp];
-  const int g = m.grp_off + item0 + wave;
-  const GroupRec gr = groups[g];
+  if (wave >= sl.cum[sl.n]) return;
+  int e = 0;
+  while (wave >= sl.cum[e + 1]) ++e;
+  const int cp = sl.cp[e];
+  const CPMeta m = sl.cps[sl.tb[e]][cp];
+  const int g = m.grp_off + sl.item0[e] + (wave - sl.cum[e]);
+  const GroupRec gr = sl.groups[sl.tb[e]][g];
+  const int nres = sl.nres[e];
+  unsigned* const ovf = ovf_all + cp;
   const int n = gr.n, ublk = gr.ublk;
   const int uni_start = m.flip ? m.bra_start : m.ket_start;
   const int sw_start = m.flip ? m.ket_start : m.bra_start;
@@ -799,6 +816,7 @@ struct cq_ctx {
   std::vector<unsigned long long> last_counts;
   // profiling: events around every class-pair kernel of every contraction
   bool profiling = false;
+  bool fused_screen = true;   // one screening launch for the class pairs screened once per call
   std::vector<int> launch_order;
   std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof_events, prof_screen_events;
   std::vector<double> prof_ms, prof_screen_ms;
@@ -1399,9 +1417,9 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->d_Npair.ensure(std::max(1, npair));
   c->last_counts.assign(kNumClassPairs, 0);
   c->prof_events.assign(kNumClassPairs, {});
-  c->prof_screen_events.assign(kNumClassPairs, {});
-  c->prof_screen_ms.assign(kNumClassPairs, 0.0);
-  c->prof_screen_launches.assign(kNumClassPairs, 0);
+  c->prof_screen_events.assign(kNumClassPairs + 1, {});   // the last: the fused screening launch
+  c->prof_screen_ms.assign(kNumClassPairs + 1, 0.0);
+  c->prof_screen_launches.assign(kNumClassPairs + 1, 0);
   c->launch_order.resize(kNumClassPairs);
   std::iota(c->launch_order.begin(), c->launch_order.end(), 0);
   std::stable_sort(c->launch_order.begin(), c->launch_order.end(),
@@ -1532,6 +1550,61 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(WG), 0, s, zr);
       HIPCHK(hipGetLastError());
     }
+  // screening launches: the class pairs' entries over groups (their plans' tables), into the
+  // task regions of this round's bound window [lo, hi)
+  auto screen_list_base = [&]() {
+    ScreenList sl{};
+    sl.cps[0] = c->d_cps[0].p;
+    sl.cps[1] = c->d_cps[1].p;
+    sl.groups[0] = c->d_groups[0].p;
+    sl.groups[1] = c->d_groups[1].p;
+    return sl;
+  };
+  auto add_screen_entry = [&](ScreenList& sl, int k, int tb, int g0, int ng, int nres) {
+    sl.cp[sl.n] = k;
+    sl.tb[sl.n] = tb;
+    sl.item0[sl.n] = g0;
+    sl.nres[sl.n] = nres;
+    sl.cum[sl.n + 1] = sl.cum[sl.n] + ng;
+    ++sl.n;
+  };
+  auto launch_screen = [&](const ScreenList& sl, hipStream_t st, std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs,
+                           bool count) {
+    const int waves = sl.cum[sl.n];
+    if (waves == 0) return;
+    hipEvent_t q0 = nullptr, q1 = nullptr;
+    if (profiling) {
+      q0 = c->get_event();
+      q1 = c->get_event();
+      HIPCHK(hipEventRecord(q0, st));
+    }
+    hipLaunchKernelGGL(screen_kernel, dim3(grid_for(waves, WG / 64)), dim3(WG), 0, st, sl, c->d_uorder.p,
+                       c->d_s12.p, c->d_pairQ.p, c->d_grp.p, c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p,
+                       c->nmol, c->d_Nmax.p, c->d_Npair.p, c->d_nglob.p, ns, lo, hi, c->d_pair_nblk.p,
+                       c->d_tasks.p, c->d_qcounts.p + (fp32 ? kNumClassPairs : 0), c->d_segs.p, c->d_ovf.p,
+                       count ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    if (profiling) {
+      HIPCHK(hipEventRecord(q1, st));
+      evs.push_back({q0, q1});
+    }
+  };
+  // every class pair screened once per call (one batch) in one launch on s, before the fork:
+  // thousands of one-group wavefronts of all class pairs fill the device together
+  std::vector<char> fused(kNumClassPairs, 0);
+  if (c->fused_screen) {
+    ScreenList sl = screen_list_base();
+    for (int k = 0; k < kNumClassPairs; ++k) {
+      const int tb = variant_of(k) >= 2 ? 1 : 0;
+      const CPMeta& m = c->cps[tb][k];
+      if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
+      const std::vector<int>& bs = c->batch_start[tb][k];
+      if (bs.size() != 2) continue;
+      add_screen_entry(sl, k, tb, bs[0], bs[1] - bs[0], c->batch_slots[tb][k][0]);
+      fused[k] = 1;
+    }
+    launch_screen(sl, s, c->prof_screen_events[kNumClassPairs], true);
+  }
   // 4. per class pair (on the auxiliary streams, highest angular momentum first: those are the
   //    latency-bound ones): screening into its task regions, then ERI + J/K digestion, one
   //    density slot per pass (the lists are screened once and kept for every slot when the class
@@ -1588,25 +1661,10 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
           hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, sk, zb);
           HIPCHK(hipGetLastError());
         }
-        if (s0 == 0 || nbatch > 1) {
-          hipEvent_t q0 = nullptr, q1 = nullptr;
-          if (profiling) {
-            q0 = c->get_event();
-            q1 = c->get_event();
-            HIPCHK(hipEventRecord(q0, sk));
-          }
-          hipLaunchKernelGGL(screen_kernel, dim3(grid_for(ng, WG / 64)), dim3(WG), 0, sk, c->d_cps[tb].p, k, g0, ng,
-                             c->d_groups[tb].p, c->d_uorder.p, c->d_s12.p, c->d_pairQ.p, c->d_grp.p,
-                             c->d_cls_grp_off.p, c->d_pair_mol.p, c->d_cmg.p, c->nmol, c->d_Nmax.p, c->d_Npair.p,
-                             c->d_nglob.p, ns,
-                             lo, hi, c->d_pair_nblk.p, c->d_tasks.p, c->d_qcounts.p + (fp32 ? kNumClassPairs : 0),
-                             c->d_segs.p, c->d_ovf.p + k,
-                             nslots, s0 == 0 ? 1 : 0);
-          HIPCHK(hipGetLastError());
-          if (profiling) {
-            HIPCHK(hipEventRecord(q1, sk));
-            c->prof_screen_events[k].push_back({q0, q1});
-          }
+        if ((s0 == 0 && !fused[k]) || nbatch > 1) {
+          ScreenList sl = screen_list_base();
+          add_screen_entry(sl, k, tb, g0, ng, nslots);
+          launch_screen(sl, sk, c->prof_screen_events[k], s0 == 0);
         }
         // profiling times the ERI + digestion kernel of each batch (not the screen)
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2090,6 +2148,12 @@ int cq_set_launch_streams(cq_ctx_t* c, int n) {
   });
 }
 
+// One screening launch for every class pair that screens once per call (on), or one per
+// class pair on its own stream (off).  The surviving quartets are the same either way.
+int cq_set_screen_fusion(cq_ctx_t* c, int on) {
+  return guarded(c, [&] { c->fused_screen = on != 0; });
+}
+
 // Per-class-pair kernel timing with HIP events recorded on the launch stream.
 int cq_set_profiling(cq_ctx_t* c, int on) {
   return guarded(c, [&] {
@@ -2120,10 +2184,9 @@ void collect_times(cq_ctx* c) {
     }
     evs.clear();
   };
-  for (int k = 0; k < kNumClassPairs; ++k) {
-    fold(c->prof_events[k], c->prof_ms[k], c->prof_launches[k]);
+  for (int k = 0; k < kNumClassPairs; ++k) fold(c->prof_events[k], c->prof_ms[k], c->prof_launches[k]);
+  for (int k = 0; k <= kNumClassPairs; ++k)
     fold(c->prof_screen_events[k], c->prof_screen_ms[k], c->prof_screen_launches[k]);
-  }
   c->event_next = 0;
 }
 }  // namespace
@@ -2139,11 +2202,13 @@ int cq_get_kernel_times(cq_ctx_t* c, double* ms, long long* launches, int max_n)
 }
 
 // Accumulated screening-kernel milliseconds and launch counts per class pair since profiling
-// was switched on (HIP events around each screen_kernel launch on its stream).
+// was switched on (HIP events around each screen_kernel launch on its stream); entry
+// CQ_NUM_CLASS_PAIRS, when max_n leaves room for it, is the fused launch of all class pairs
+// that screen once per call.
 int cq_get_screen_times(cq_ctx_t* c, double* ms, long long* launches, int max_n) {
   return guarded(c, [&] {
     collect_times(c);
-    for (int k = 0; k < kNumClassPairs && k < max_n; ++k) {
+    for (int k = 0; k <= kNumClassPairs && k < max_n; ++k) {
       ms[k] = c->prof_screen_ms[k];
       launches[k] = c->prof_screen_launches[k];
     }

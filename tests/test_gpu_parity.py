@@ -348,3 +348,42 @@ def test_group_tiles_match_oracle(max_group, union_bound, lds):
         ao.set_class_tile(0, 17, 0, 0)
     with pytest.raises(_lib.CQError):
         ao.set_class_tile(0, 1, 32, 0)
+
+
+def test_screen_fusion_matches_per_class_screening():
+    """All class pairs screened in one launch (the default) or one launch per class pair on its
+    stream, with a budget that leaves some class pairs in several batches (those screen per
+    batch) and without: J and K against the oracle, identical surviving counts, for two
+    density slots (UHF-style) and in the mixed-precision rounds ((H2O)8 / cc-pVDZ)."""
+    from chronusq_public_amd import _lib
+    L = _lib.load()
+    mol = M.water_cluster(8, seed=11, box_ang=M.scaled_box(8))
+    bs, ao, ob = _setup(mol, "cc-pvdz")
+    X = _sym_random(bs.nbasis, 17)
+    Y = np.asfortranarray(0.3 * _sym_random(bs.nbasis, 18))
+    Qg = ao.computeSchwartz()
+    (Jo, Ko, Ky), n = _oracle_jk(ob, Qg, [X, X, Y], [0, 1, 1], [1.0, -0.5, -0.5], 1e-12)
+    for budget in (None, 60000):
+        if budget:
+            ao.set_task_budget(budget)
+            assert ao.task_plan()[1] > 1
+        for fusion in (1, 0):
+            print(f"[fusion] budget={budget} fusion={fusion}", flush=True)
+            _lib.check(L.cq_set_screen_fusion(ao._h, fusion), ao._h)
+            J, K, KY = _gpu_jk(ao, [X, X, Y], [COULOMB, EXCHANGE, EXCHANGE], [1.0, -0.5, -0.5], 1e-12)
+            assert sum(ao.quartet_counts().values()) == n
+            assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL and np.abs(KY - Ky).max() < TOL
+    # mixed precision: the FP32 window's round screens again (fused or not): same counts
+    ao.setPrecision("mixed", 1e-5)
+    res = []
+    try:
+        for fusion in (1, 0):
+            _lib.check(L.cq_set_screen_fusion(ao._h, fusion), ao._h)
+            J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+            res.append((J, K, ao.precision_counts()))
+            for a, b in ((J, Jo), (K, Ko)):
+                assert np.abs(a - b).max() < 1e-9 + 1e-6 * np.abs(b).max()
+    finally:
+        ao.setPrecision("fp64")
+    assert res[0][2] == res[1][2] and res[0][2][1] > 0
+    assert np.abs(res[0][0] - res[1][0]).max() < 1e-12 and np.abs(res[0][1] - res[1][1]).max() < 1e-12
