@@ -923,14 +923,27 @@ size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int u
 // Default union bound per class pair (kernel orientation), from a sweep of 256..1024 with
 // single-pair groups on (H2O)16/cc-pVDZ, tau = 1e-12 (per-class kernel + screening time): the
 // bound trades fewer K-column flushes (large) against more work items to balance (small).
+// Built-in tile of each class pair: the largest group (uniform pairs sharing a shell) and the
+// union bound (swept pairs per task region).  Measured on (H2O)16/cc-pVDZ with screening in one
+// launch: groups of 2-4 members pay where a swept pair's J slot would otherwise be flushed to
+// HBM once per uniform pair (low angular momentum, thousands of uniform pairs); elsewhere
+// single-pair groups are as fast.
+struct TileDefault { const char* name; int max_group, ub; };
+static const TileDefault kTileDefaults[] = {
+    {"ssss", 1, 512}, {"ssps", 4, 256}, {"psps", 4, 512}, {"sspp", 2, 512}, {"ppps", 4, 512},
+    {"pppp", 1, 256}, {"ssds", 1, 512}, {"psds", 3, 512}, {"ppds", 1, 256}, {"dsds", 1, 256},
+    {"ssdp", 1, 512}, {"dpps", 1, 512}, {"dppp", 1, 1024}, {"dpds", 1, 256}, {"dpdp", 1, 512},
+    {"ddss", 1, 256}, {"ddps", 1, 512}, {"ddpp", 1, 512}, {"ddds", 1, 512}, {"dddp", 1, 512},
+    {"dddd", 1, 256}};
+
+const TileDefault* tile_default(const ClassKernels& ck) {
+  for (const auto& t : kTileDefaults)
+    if (std::strcmp(t.name, ck.name) == 0) return &t;
+  return nullptr;
+}
+
 int default_union_bound(const ClassKernels& ck, size_t jside) {
-  static const struct { const char* name; int ub; } tab[] = {
-      {"ssss", 512}, {"ssps", 1024}, {"psps", 1024}, {"sspp", 1024}, {"ppps", 512}, {"pppp", 256},
-      {"ssds", 1024}, {"psds", 1024}, {"ppds", 256}, {"dsds", 256}, {"ssdp", 512}, {"dpps", 512},
-      {"dppp", 1024}, {"dpds", 256}, {"dpdp", 512}, {"ddss", 256}, {"ddps", 512}, {"ddpp", 512},
-      {"ddds", 512}, {"dddp", 512}, {"dddd", 256}};
-  for (const auto& t : tab)
-    if (std::strcmp(t.name, ck.name) == 0) return t.ub;
+  if (const TileDefault* t = tile_default(ck)) return t->ub;
   return (int)std::min<size_t>(UNION_MAX, std::max<size_t>(64, (16 * 1024) / (jside * sizeof(double)) / 64 * 64));
 }
 
@@ -944,9 +957,8 @@ void class_tile(const cq_ctx* c, int k, int variant, int& mmax, int& ub) {
   // a union bound whose J slots alone overflow the device's LDS is lowered (multiples of 64)
   while (ub > 64 && group_lds(ck, variant, c->lds_n, 1, ub) > c->max_dyn_lds) ub -= 64;
   mmax = 0;
-  // single-pair groups by default: on (H2O)16/cc-pVDZ larger groups cut the J updates but cost
-  // as much again in screening (every member is tested against the group's whole sweep)
-  const int cap = c->tile_mmax[k] > 0 ? std::min(c->tile_mmax[k], MAX_GROUP) : 1;
+  const TileDefault* t = tile_default(ck);
+  const int cap = c->tile_mmax[k] > 0 ? std::min(c->tile_mmax[k], MAX_GROUP) : (t ? t->max_group : 1);
   for (int mm = cap; mm >= 1; --mm)
     if (group_lds(ck, variant, c->lds_n, mm, ub) <= limit) { mmax = mm; break; }
   if (mmax == 0 && group_lds(ck, variant, c->lds_n, 1, ub) <= c->max_dyn_lds) mmax = 1;
@@ -1132,6 +1144,8 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   }
   c->nsph = sph;
   c->ncart = cart;
+  // Cartesian J/K accumulators of ncart^2 elements, addressed with 32-bit offsets in the kernels
+  if (cart > 46340) throw std::invalid_argument("a context holds at most 46340 Cartesian functions");
   c->max_dyn_lds = device_dyn_lds();
   // molecules: consecutive shells, ids 0, 1, ... in order (nullptr: one molecule)
   c->shell_mol.assign(nshell, 0);

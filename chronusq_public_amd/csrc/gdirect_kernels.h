@@ -114,6 +114,16 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
   const bool sw = P.a != P.s1;
   return sw ? (long)(P.b0 + j) * N + P.a0 + i : (long)(P.a0 + i) * N + P.b0 + j;
 }
+// The same address kept in one int while a swept pair's J slot fills: element (0, 0) of the
+// block, bitwise negated when the pair is held swapped (the context keeps N^2 < 2^31).
+__device__ __forceinline__ int jcode(const PairInfo& P, int N) {
+  const bool sw = P.a != P.s1;
+  const int a00 = sw ? P.b0 * N + P.a0 : P.a0 * N + P.b0;
+  return sw ? ~a00 : a00;
+}
+__device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
+  return code < 0 ? (long)(~code) + (long)j * N + i : (long)code + (long)i * N + j;
+}
 
 // Group kernels.  Work items are (task region, chunk) pairs from an atomic queue; each workgroup
 // runs the region's members one after the other (the member wavefront-uniform, one quartet per
@@ -124,7 +134,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
 //  * bra-keyed (eri_grpb, DIG 2): K rows of the shared a shell and of every member's b shell,
 //    J(a, b_m) of every member, and J of every union ket.
 // Dynamic LDS, ket-keyed: [(NC + mmax ND) LN K columns (DIG 1)][mmax NC ND J(c,d)][ub NA NB J slots]
-//                         [ub ints: the swept pair of each slot]
+//                         [ub ints: the J address code of each slot's swept pair]
 //              bra-keyed: [(NA + mmax NB) LN K rows][mmax NA NB J(a,b)][ub NC ND J slots][ub ints]
 #define DEFINE_CLASS_KERNELS(NAME, LTOT)                                                        \
   template <int WGS, int DIG, typename R>                                                     \
@@ -175,10 +185,10 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
         const bool active = t < E;                                                            \
         const int pos = e0 + (active ? t : t0 + wofs);                                        \
         const uint2 e = reg[pos];                                                             \
-        if (active) spair[e.y] = (int)pair_of(e.x);                                           \
         int mi = 0;                                                                           \
         for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                      \
         const PairInfo B = pairs[pair_of(e.x)];                                               \
+        if (active) spair[e.y] = jcode(B, N);                                                 \
         bool pending = active;                                                                \
         while (true) {                                                                        \
           const unsigned long long pm = __ballot(pending);                                    \
@@ -219,7 +229,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
         const double v = jsl[k];                                                              \
         if (v != 0.0) {                                                                       \
           const int u = k / NAB, ia = (k - u * NAB) / NB, ib = k - u * NAB - ia * NB;         \
-          atomicAdd(&jk.GJ[0][jaddr(pairs[spair[u]], N, ia, ib)], v);                         \
+          atomicAdd(&jk.GJ[0][jcode_addr(spair[u], N, ia, ib)], v);                           \
           jsl[k] = 0.0;                                                                       \
         }                                                                                     \
       }                                                                                       \
@@ -300,10 +310,10 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
         const bool active = t < E;                                                              \
         const int pos = e0 + (active ? t : t0 + wofs);                                          \
         const uint2 e = reg[pos];                                                               \
-        if (active) spair[e.y] = (int)pair_of(e.x);                                             \
         int mi = 0;                                                                             \
         for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                        \
         const PairInfo K = pairs[pair_of(e.x)];                                                 \
+        if (active) spair[e.y] = jcode(K, N);                                                   \
         bool pending = active;                                                                  \
         while (true) {                                                                          \
           const unsigned long long pm = __ballot(pending);                                      \
@@ -342,7 +352,7 @@ __device__ __forceinline__ long jaddr(const PairInfo& P, int N, int i, int j) {
         const double v = jsl[k];                                                                \
         if (v != 0.0) {                                                                         \
           const int u = k / NCD, ic = (k - u * NCD) / ND, idd = k - u * NCD - ic * ND;          \
-          atomicAdd(&jk.GJ[0][jaddr(pairs[spair[u]], N, ic, idd)], v);                          \
+          atomicAdd(&jk.GJ[0][jcode_addr(spair[u], N, ic, idd)], v);                            \
           jsl[k] = 0.0;                                                                         \
         }                                                                                       \
       }                                                                                         \
