@@ -26,7 +26,7 @@ EXPORTED = [
     "cq_get_schwarz_threshold", "cq_set_precision_mode", "cq_get_dims", "cq_get_schwarz",
     "cq_contract_direct", "cq_contract_direct_device", "cq_contract_direct_device_noalloc", "cq_get_quartet_counts", "cq_get_precision_counts",
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
-    "cq_get_kernel_times", "cq_get_screen_times", "cq_set_screen_fusion", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_set_launch_streams", "cq_set_lds_digestion",
+    "cq_get_kernel_times", "cq_get_screen_times", "cq_set_screen_fusion", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_multipole", "cq_set_launch_streams", "cq_set_lds_digestion",
     "cq_solid_harmonics", "cq_set_class_digestion", "cq_get_class_digestion",
     "cq_set_class_stream", "cq_set_task_budget", "cq_get_task_plan", "cq_set_class_tile", "cq_get_class_tile",
     # learned initial guess (include/cqgpu_guess.h)
@@ -84,6 +84,7 @@ def load():
     L.cq_set_lds_digestion.argtypes = [vp, ctypes.c_int]
     L.cq_one_electron.argtypes = [vp, ctypes.c_int, dp, dp, vp, vp, vp, ctypes.c_int, vp]
     L.cq_one_electron_batch.argtypes = [vp, ctypes.c_int, dp, dp, ip, vp, vp, vp, ctypes.c_int, vp]
+    L.cq_multipole.argtypes = [vp, ctypes.c_int, dp, vp, ctypes.c_int, vp]
     L.cq_get_kernel_times.argtypes = [vp, dp, P(ctypes.c_longlong), ctypes.c_int]
     L.cq_get_screen_times.argtypes = [vp, dp, P(ctypes.c_longlong), ctypes.c_int]
     L.cq_get_work_stats.argtypes = [vp, dp, ctypes.c_int]

@@ -284,6 +284,25 @@ class AOIntegrals:
                                           N, ctypes.c_void_p(stream)), self._h)
         return tuple(out)
 
+    def computeMultipole(self, order: int = 3, origin=None, device=None):
+        """Electric multipole integrals <mu| x^i y^j z^k |nu> about `origin` (bohr; default the
+        origin, as the reference's libint2 emultipole engines), 1 <= i + j + k <= order: a list
+        of 3, 9 or 19 symmetric N x N float64 tensors on the GPU in the order x, y, z; xx, xy,
+        xz, yy, yz, zz; xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz (cq_multipole).
+        AOIntegrals::OneEDriver with emultipole1..3, src/aointegrals/aointegrals_builders.cpp:38-55."""
+        import torch
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        N = self.nBasis
+        nops = {1: 3, 2: 9, 3: 19}.get(order)
+        if nops is None:
+            raise ValueError("order must be 1, 2 or 3")
+        out = torch.empty((nops, N, N), dtype=torch.float64, device=dev)
+        C = None if origin is None else np.ascontiguousarray(origin, dtype=np.float64)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        check(_lib.load().cq_multipole(self._h, int(order), _dp(C) if C is not None else None,
+                                       ctypes.c_void_p(out.data_ptr()), N, ctypes.c_void_p(stream)), self._h)
+        return list(out.unbind(0))
+
     def computeAOOneE_batch(self, mols, device=None):
         """S, T, V of a batch context (AOIntegrals.batch) for the molecules in the same order:
         N x N tensors holding every molecule's own integrals in its diagonal block

@@ -1816,6 +1816,39 @@ void one_electron(cq_ctx* c, int natom, const double* Z, const double* xyz, cons
   d_mat0.release();
 }
 
+// Cartesian multipole integrals (orders 1 .. order) about `origin`, transformed to the shells'
+// basis: 3, 9 or 19 column-major n x n matrices (leading dimension ld) back to back in `out`.
+// Synchronous.
+void multipole(cq_ctx* c, int order, const double* origin, double* out, int ld, void* stream) {
+  if (order < 1 || order > 3 || !out || ld < c->nsph)
+    throw std::invalid_argument("multipole: order must be 1, 2 or 3, with an output of leading dimension >= n");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int nops = order == 1 ? 3 : (order == 2 ? 9 : 19);
+  const double C[3] = {origin ? origin[0] : 0.0, origin ? origin[1] : 0.0, origin ? origin[2] : 0.0};
+  const long nc2 = (long)c->ncart * c->ncart;
+  c->d_onee.ensure((size_t)nops * nc2);
+  double* Mc = c->d_onee.p;
+  ZeroList zl{};
+  zl.p[0] = (unsigned*)Mc;
+  zl.n[0] = nops * nc2 * 2;
+  zl.count = 1;
+  hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(WG), 0, s, zl);
+  OneEShellData sd{c->d_prim_off.p, c->d_alpha.p, c->d_coef.p, c->d_origin.p};
+  int max_prim = 1;
+  for (size_t k = 0; k + 1 < c->prim_off_host.size(); ++k)
+    max_prim = std::max(max_prim, c->prim_off_host[k + 1] - c->prim_off_host[k]);
+  for (int X = 0; X < c->npc; ++X)
+    launch_multipole(X, c->d_pairs_1e.p, c->class_start_1e[X], c->class_start_1e[X + 1], sd, max_prim, C, order,
+                     c->ncart, Mc, s);
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < nops; ++k)
+    hipLaunchKernelGGL(sph_transform_kernel, dim3(grid_for((long)c->nsph * c->nsph, WG)), dim3(WG), 0, s,
+                       Mc + k * nc2, out + (long)k * ld * c->nsph, c->nsph, ld, c->d_shells.p, c->d_sph2shell.p,
+                       c->ncart, c->d_tco.p, c->d_tco_off.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+}
+
 }  // namespace
 
 // =======================================================================================
@@ -2026,6 +2059,10 @@ int cq_one_electron_batch(cq_ctx_t* c, int natom, const double* Z, const double*
     if (!atom_molecule) throw std::invalid_argument("atom_molecule is required");
     one_electron(c, natom, Z, xyz, atom_molecule, S, T, V, ld, stream);
   });
+}
+
+int cq_multipole(cq_ctx_t* c, int order, const double* origin, double* out, int ld, void* stream) {
+  return guarded(c, [&] { multipole(c, order, origin, out, ld, stream); });
 }
 
 // Selects the J/K digestion: 1 (default) accumulates K columns and J(3,4) of each ket pair in

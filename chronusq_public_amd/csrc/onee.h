@@ -23,3 +23,9 @@ struct OneEShellData {
 void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
                  int max_prim, const double* Z, const double* xyz, int natom, int max_atoms, const double* boys,
                  int ncart, double* S, double* T, double* V, hipStream_t stream);
+
+// Multipole moment integrals about C[3] (host array, bohr) of one pair class: order 1 .. 3 gives
+// the 3, 9 or 19 Cartesian operators x, y, z; xx, xy, xz, yy, yz, zz; xxx, xxy, xxz, xyy, xyz,
+// xzz, yyy, yyz, yzz, zzz, each added into its own zeroed row-major ncart x ncart block of M.
+void launch_multipole(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
+                      int max_prim, const double* C, int order, int ncart, double* M, hipStream_t stream);

@@ -1,6 +1,6 @@
-// One-electron integrals on the GPU: overlap S, kinetic T, nuclear attraction V
-// (replaces AOIntegrals::computeAOOneE / OneEDriver, src/aointegrals/aointegrals_builders.cpp:
-// 27-132,337-467).  McMurchie-Davidson: Hermite expansion coefficients by their closed form
+// One-electron integrals on the GPU: overlap S, kinetic T, nuclear attraction V, and the
+// Cartesian multipoles up to third order (replaces AOIntegrals::computeAOOneE / OneEDriver,
+// src/aointegrals/aointegrals_builders.cpp:27-132,337-467).  McMurchie-Davidson: Hermite expansion coefficients by their closed form
 //   E^{ij}_t = sum_{k<=i,l<=j} C(i,k) C(j,l) PA^{i-k} PB^{j-l} n!/(t! m!) (1/4p)^m (1/2p)^t,
 //   n = k + l, m = (n - t)/2,
 // S_1D(i,j) = E^{ij}_0 sqrt(pi/p),
@@ -212,7 +212,98 @@ __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ p
     }
 }
 
+// Cartesian multipole x_C^EX y_C^EY z_C^EZ about C: per direction
+//   <i| x_C^e |j> = sum_k C(e,k) (B - C)^(e-k) <i| j + k>     (x - C = (x - B) + (B - C)),
+// the overlaps with raised angular momentum on b from the Hermite coefficients E^{i,j+k}_0.
+template <int E>
+__device__ __forceinline__ double moment_1d(int i, int j, double PA, double PB, double o2p, double BC) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k <= E; ++k) s += cbinom(E, k) * ipow(BC, E - k) * hermE(i, j + k, 0, PA, PB, o2p);
+  return s;
+}
+
+template <int LA, int LB, int EX, int EY, int EZ>
+__global__ void __launch_bounds__(WG) multipole_kernel(const PairInfo* __restrict__ pairs, int p_begin, int p_end,
+                                                       const OneEShellData sd, double Cx, double Cy, double Cz,
+                                                       int ncart, double* __restrict__ M) {
+  constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
+  const int P = p_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= p_end) return;
+  const PairInfo pi = pairs[P];
+  const int a = pi.a, b = pi.b;
+  const int ip = sd.prim_off[a] + (int)blockIdx.y;   // slice: one primitive of a
+  if (ip >= sd.prim_off[a + 1]) return;
+  const double A[3] = {sd.origin[3 * a], sd.origin[3 * a + 1], sd.origin[3 * a + 2]};
+  const double B[3] = {sd.origin[3 * b], sd.origin[3 * b + 1], sd.origin[3 * b + 2]};
+  const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+  const double BCx = B[0] - Cx, BCy = B[1] - Cy, BCz = B[2] - Cz;
+  double m[NA][NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) m[i][j] = 0.0;
+  for (int jp = sd.prim_off[b]; jp < sd.prim_off[b + 1]; ++jp) {
+    const double al = sd.alpha[ip], be = sd.alpha[jp];
+    const double p = al + be, o2p = 0.5 / p;
+    const double sq = sqrt(3.14159265358979323846 / p);
+    const double K = exp(-al * be / p * AB2) * sd.coef[ip] * sd.coef[jp] * sq * sq * sq;
+    const double PA0 = be / p * (B[0] - A[0]), PA1 = be / p * (B[1] - A[1]), PA2 = be / p * (B[2] - A[2]);
+    const double PB0 = al / p * (A[0] - B[0]), PB1 = al / p * (A[1] - B[1]), PB2 = al / p * (A[2] - B[2]);
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) {
+        const int ax = comp_x(LA, ia), ay = comp_y(LA, ia), az = comp_z(LA, ia);
+        const int bx = comp_x(LB, ib), by = comp_y(LB, ib), bz = comp_z(LB, ib);
+        m[ia][ib] += K * moment_1d<EX>(ax, bx, PA0, PB0, o2p, BCx) * moment_1d<EY>(ay, by, PA1, PB1, o2p, BCy) *
+                     moment_1d<EZ>(az, bz, PA2, PB2, o2p, BCz);
+      }
+  }
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      const long r = pi.a0 + ia, cidx = pi.b0 + ib;
+      atomicAdd(&M[r * ncart + cidx], m[ia][ib]);
+      if (a != b) atomicAdd(&M[cidx * ncart + r], m[ia][ib]);
+    }
+}
+
+template <int LA, int LB>
+void launch_multipole_class(const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd, int max_prim,
+                            const double* C, int order, int ncart, double* M, hipStream_t stream) {
+  const int n = p_end - p_begin;
+  const dim3 grid((n + WG - 1) / WG, std::max(1, max_prim));
+  const long nc2 = (long)ncart * ncart;
+  int k = 0;
+#define CQ_MP(EX, EY, EZ)                                                                                   \
+  hipLaunchKernelGGL((multipole_kernel<LA, LB, EX, EY, EZ>), grid, dim3(WG), 0, stream, pairs, p_begin, p_end, \
+                     sd, C[0], C[1], C[2], ncart, M + (k++) * nc2)
+  if (order >= 1) { CQ_MP(1, 0, 0); CQ_MP(0, 1, 0); CQ_MP(0, 0, 1); }
+  if (order >= 2) { CQ_MP(2, 0, 0); CQ_MP(1, 1, 0); CQ_MP(1, 0, 1); CQ_MP(0, 2, 0); CQ_MP(0, 1, 1); CQ_MP(0, 0, 2); }
+  if (order >= 3) {
+    CQ_MP(3, 0, 0); CQ_MP(2, 1, 0); CQ_MP(2, 0, 1); CQ_MP(1, 2, 0); CQ_MP(1, 1, 1);
+    CQ_MP(1, 0, 2); CQ_MP(0, 3, 0); CQ_MP(0, 2, 1); CQ_MP(0, 1, 2); CQ_MP(0, 0, 3);
+  }
+#undef CQ_MP
+}
+
 }  // namespace
+
+void launch_multipole(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
+                      int max_prim, const double* C, int order, int ncart, double* M, hipStream_t stream) {
+  if (p_end <= p_begin) return;
+  switch (pair_class) {
+    case 0: launch_multipole_class<0, 0>(pairs, p_begin, p_end, sd, max_prim, C, order, ncart, M, stream); break;
+    case 1: launch_multipole_class<1, 0>(pairs, p_begin, p_end, sd, max_prim, C, order, ncart, M, stream); break;
+    case 2: launch_multipole_class<1, 1>(pairs, p_begin, p_end, sd, max_prim, C, order, ncart, M, stream); break;
+    case 3: launch_multipole_class<2, 0>(pairs, p_begin, p_end, sd, max_prim, C, order, ncart, M, stream); break;
+    case 4: launch_multipole_class<2, 1>(pairs, p_begin, p_end, sd, max_prim, C, order, ncart, M, stream); break;
+    case 5: launch_multipole_class<2, 2>(pairs, p_begin, p_end, sd, max_prim, C, order, ncart, M, stream); break;
+    default: break;
+  }
+}
 
 void launch_onee(int pair_class, const PairInfo* pairs, int p_begin, int p_end, const OneEShellData& sd,
                  int max_prim, const double* Z, const double* xyz, int natom, int max_atoms, const double* boys,

@@ -244,6 +244,42 @@ class SingleSlater:
             E += 0.25 * float((Mz * PTz).sum())
         return E
 
+    # ------------------------------------------------------------------ properties
+    # units of the reference's printed and golden multipoles (tests/rununit.py:312-313,
+    # src/python/src/.chronusq_old.py.in:222): e bohr per Debye, Angstrom per bohr
+    AU_PER_DEBYE = 0.3934303070
+    ANGSTROM_PER_BOHR = 0.5291772083
+
+    def computeMultipole(self, P, order: int = 3) -> Dict[str, np.ndarray]:
+        """Dipole (Debye), quadrupole (Debye Angstrom, 3 x 3) and octupole (Debye Angstrom^2,
+        3 x 3 x 3) of the total density P about the origin: SingleSlater::computeMultipole
+        (include/singleslater/singleslater_properties.h:115-172) with the electronic parts of
+        include/quantum/quantum_stdproperties.h:25-66 -- minus Tr(P r_i ...) over the GPU
+        multipole integrals, plus sum_A Z_A R_Ai ... -- in the units of the reference's golden
+        values.  The quadrupole is the plain second moment (not traceless), as compared there."""
+        torch = self.torch
+        M = self.ao.computeMultipole(order, None, self.dev)
+        el = torch.stack([(P * m).sum() for m in M]).cpu().numpy()
+        Z = self.mol.Z.astype(np.float64)
+        R = self.mol.coords_bohr
+        out: Dict[str, np.ndarray] = {}
+        dip = -el[0:3] + Z @ R
+        out["dipole"] = dip / self.AU_PER_DEBYE
+        if order >= 2:
+            q = np.zeros((3, 3))
+            for k, (i, j) in enumerate([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+                q[i, j] = q[j, i] = -el[3 + k] + float(np.sum(Z * R[:, i] * R[:, j]))
+            out["quadrupole"] = q * self.ANGSTROM_PER_BOHR / self.AU_PER_DEBYE
+        if order >= 3:
+            o = np.zeros((3, 3, 3))
+            idx = [(i, j, k) for i in range(3) for j in range(i, 3) for k in range(j, 3)]
+            for n, (i, j, k) in enumerate(idx):
+                v = -el[9 + n] + float(np.sum(Z * R[:, i] * R[:, j] * R[:, k]))
+                for a, b, c in {(i, j, k), (i, k, j), (j, i, k), (j, k, i), (k, i, j), (k, j, i)}:
+                    o[a, b, c] = v
+            out["octupole"] = o * self.ANGSTROM_PER_BOHR ** 2 / self.AU_PER_DEBYE
+        return out
+
     def core_guess(self):
         na, nb = self.nocc
         Pa, _ = self._diag_density(self.H, na)

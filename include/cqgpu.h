@@ -124,6 +124,15 @@ int cq_one_electron(cq_ctx_t* ctx, int natom, const double* Z, const double* xyz
 int cq_one_electron_batch(cq_ctx_t* ctx, int natom, const double* Z, const double* xyz, const int* atom_molecule,
                           double* S, double* T, double* V, int ld, void* stream);
 
+/* Electric multipole integrals <mu| x^i y^j z^k |nu> about origin[3] (host, bohr; NULL: the
+ * origin), 1 <= i + j + k <= order <= 3, in the shells' basis: 3, 9 or 19 column-major n x n
+ * matrices (leading dimension ld) back to back in device memory `out`, in the order x, y, z;
+ * xx, xy, xz, yy, yz, zz; xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz.  Replaces
+ * AOIntegrals::OneEDriver with libint2::Operator::emultipole1..3 (origin 0; the reference keeps
+ * the overlap as its first block: cq_one_electron gives it), src/aointegrals/
+ * aointegrals_builders.cpp:38-55,387-392.  Synchronous. */
+int cq_multipole(cq_ctx_t* ctx, int order, const double* origin, double* out, int ld, void* stream);
+
 /* Real solid-harmonic coefficients for angular momentum l (0..2): (2l+1) x ncart(l) row-major,
  * m = -l..l, in the shell's uniformly normalised Cartesian functions (x^l ... z^l) -- the
  * transform of every pure shell (cart2sphCoeff, src/basisset/basisset_read.cpp:871-990). */

@@ -63,6 +63,7 @@ def lib():
         L.orc_count_screened.restype = ctypes.c_long
         L.orc_count_screened.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
         L.orc_one_electron.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, dp, dp]
+        L.orc_multipole.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -166,6 +167,17 @@ class OracleBasis:
         xyz = np.ascontiguousarray(mol.coords_bohr, dtype=np.float64)
         lib().orc_one_electron(self.h, mol.natom, _dp(Z), _dp(xyz), _dp(S), _dp(T), _dp(V))
         return S, T, V
+
+    def multipole(self, order: int = 3, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """Multipole integrals of orders 1 .. order about origin (bohr): (3 | 9 | 19, N, N), the
+        operators x, y, z; xx, xy, xz, yy, yz, zz; xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz."""
+        nops = {1: 3, 2: 9, 3: 19}[order]
+        N = self.nbf
+        out = np.zeros(nops * N * N)
+        C = np.ascontiguousarray(origin, dtype=np.float64)
+        lib().orc_multipole(self.h, int(order), _dp(C), _dp(out))
+        # each operator's matrix is column-major; the matrices are symmetric
+        return out.reshape(nops, N, N).transpose(0, 2, 1).copy()
 
 
 def brute_force_contract(eri: np.ndarray, X: np.ndarray, ctype: int, scale: float) -> np.ndarray:

@@ -16,11 +16,14 @@
 //   * The direct contraction, loop for loop (include/aointegrals/aointegrals_contract.h:24-194).
 //   * The N^4 brute-force contraction (include/aointegrals/aointegrals_contract.h:197-262).
 //   * One-electron S, T, V (src/aointegrals/aointegrals_builders.cpp:27-132,337-467).
+//   * Multipole integrals to third order (libint2 emultipole1..3 in OneEDriver,
+//     src/aointegrals/aointegrals_builders.cpp:38-55), by Hermite moments.
 //
 // Parity status: every function is pinned by a test in tests/test_oracle.py (closed forms,
 // the second algorithm, finite differences, permutational symmetry, the reference's golden
 // SCF energies).  None is "parity unpinned".
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -491,6 +494,32 @@ void quartet(const Basis& bs, int s1, int s2, int s3, int s4, std::vector<double
 }
 
 // ---------------------------------------------------------------- one-electron (MD)
+// A Cartesian block of shells A, B (row-major na x nb) in the shells' basis, written into the
+// column-major N x N matrix dst.
+void scatter_block(const Shell& A, const Shell& B, const std::vector<LD>& src, double* dst, int N) {
+  if (!dst) return;
+  const int la = A.l, lb = B.l, na = ncart(la), nb = ncart(lb);
+  std::vector<double> blk(src.size());
+  for (size_t i = 0; i < src.size(); ++i) blk[i] = (double)src[i];
+  int dA = na, dB = nb;
+  if (A.pure) {
+    const auto& M = car2sph_cached(la); int ns = 2 * la + 1;
+    std::vector<double> o((size_t)ns * dB, 0.0);
+    for (int i = 0; i < ns; ++i) for (int c = 0; c < na; ++c) for (int j = 0; j < dB; ++j)
+      o[i * dB + j] += M[i * na + c] * blk[c * dB + j];
+    blk.swap(o); dA = ns;
+  }
+  if (B.pure) {
+    const auto& M = car2sph_cached(lb); int ns = 2 * lb + 1;
+    std::vector<double> o((size_t)dA * ns, 0.0);
+    for (int i = 0; i < dA; ++i) for (int j = 0; j < ns; ++j) for (int c = 0; c < nb; ++c)
+      o[i * ns + j] += M[j * nb + c] * blk[i * nb + c];
+    blk.swap(o); dB = ns;
+  }
+  for (int i = 0; i < dA; ++i) for (int j = 0; j < dB; ++j)
+    dst[(size_t)(B.off + j) * N + (A.off + i)] = blk[i * dB + j];
+}
+
 void overlap_kinetic_nuclear(const Basis& bs, int natom, const double* Z, const double* xyz,
                              double* S, double* T, double* V) {
   const int N = bs.nbf;
@@ -544,30 +573,64 @@ void overlap_kinetic_nuclear(const Basis& bs, int natom, const double* Z, const 
       }
     }
     // to shell basis and scatter (column-major N x N)
-    for (int which = 0; which < 3; ++which) {
-      std::vector<LD>& src = which == 0 ? s_ : which == 1 ? t_ : v_;
-      double* dst = which == 0 ? S : which == 1 ? T : V;
-      if (!dst) continue;
-      std::vector<double> blk(src.size());
-      for (size_t i = 0; i < src.size(); ++i) blk[i] = (double)src[i];
-      int dA = na, dB = nb;
-      if (A.pure) {
-        const auto& M = car2sph_cached(la); int ns = 2 * la + 1;
-        std::vector<double> o((size_t)ns * dB, 0.0);
-        for (int i = 0; i < ns; ++i) for (int c = 0; c < na; ++c) for (int j = 0; j < dB; ++j)
-          o[i * dB + j] += M[i * na + c] * blk[c * dB + j];
-        blk.swap(o); dA = ns;
+    scatter_block(A, B, s_, S, N);
+    scatter_block(A, B, t_, T, N);
+    scatter_block(A, B, v_, V, N);
+  }
+}
+
+// Cartesian multipole integrals <a| x_C^ex y_C^ey z_C^ez |b> (restating what libint2's
+// emultipole engines give AOIntegrals::OneEDriver, src/aointegrals/aointegrals_builders.cpp:
+// 38-55) by Hermite moments, a different route from the device's: per direction
+//   <i| x_C^e |j> = sum_t E^{ij}_t M^e_t,   M^0_0 = sqrt(pi / p),
+//   M^{e+1}_t = t M^e_{t-1} + X_PC M^e_t + M^e_{t+1} / (2p)
+// (Helgaker, Jorgensen, Olsen, Molecular Electronic-Structure Theory, eq. 9.5.x).  Operators of
+// orders 1 .. order in the order x, y, z; xx, xy, xz, yy, yz, zz; xxx, xxy, ..., zzz; out holds
+// one column-major N x N matrix per operator.
+void multipole_ints(const Basis& bs, int order, const double* Cin, double* out) {
+  const int N = bs.nbf;
+  const LD pi = 3.14159265358979323846264338327950288L;
+  std::vector<std::array<int, 3>> ops;
+  for (int e = 1; e <= order; ++e)
+    for (int i = e; i >= 0; --i)
+      for (int j = e - i; j >= 0; --j) ops.push_back({i, j, e - i - j});
+  const LD C[3] = {Cin[0], Cin[1], Cin[2]};
+  for (int s1 = 0; s1 < (int)bs.sh.size(); ++s1)
+  for (int s2 = 0; s2 < (int)bs.sh.size(); ++s2) {
+    const Shell& A = bs.sh[s1]; const Shell& B = bs.sh[s2];
+    const int la = A.l, lb = B.l, na = ncart(la), nb = ncart(lb);
+    std::vector<int> ax, ay, az, bx, by, bz;
+    cart_list(la, ax, ay, az); cart_list(lb, bx, by, bz);
+    std::vector<std::vector<LD>> m_(ops.size(), std::vector<LD>((size_t)na * nb, 0));
+    std::vector<LD> E[3];
+    const int TT = la + lb + 1;
+    for (int pa = 0; pa < A.nprim; ++pa) for (int pb = 0; pb < B.nprim; ++pb) {
+      const LD a = A.a[pa], b = B.a[pb], p = a + b;
+      const LD cab = (LD)A.c[pa] * (LD)B.c[pb];
+      // Hermite moments M[k][e][t], t <= e + la + lb
+      LD M[3][4][8] = {};
+      for (int k = 0; k < 3; ++k) {
+        hermite_E(la, lb, a, b, A.O[k], B.O[k], E[k]);
+        const LD XPC = (a * (LD)A.O[k] + b * (LD)B.O[k]) / p - C[k];
+        M[k][0][0] = std::sqrt(pi / p);
+        for (int e = 0; e < 3; ++e)
+          for (int t = 0; t <= e + 1; ++t) {
+            LD v = XPC * M[k][e][t] + M[k][e][t + 1] / (2 * p);
+            if (t > 0) v += t * M[k][e][t - 1];
+            M[k][e + 1][t] = v;
+          }
       }
-      if (B.pure) {
-        const auto& M = car2sph_cached(lb); int ns = 2 * lb + 1;
-        std::vector<double> o((size_t)dA * ns, 0.0);
-        for (int i = 0; i < dA; ++i) for (int j = 0; j < ns; ++j) for (int c = 0; c < nb; ++c)
-          o[i * ns + j] += M[j * nb + c] * blk[i * nb + c];
-        blk.swap(o); dB = ns;
-      }
-      for (int i = 0; i < dA; ++i) for (int j = 0; j < dB; ++j)
-        dst[(size_t)(B.off + j) * N + (A.off + i)] = blk[i * dB + j];
+      auto mom = [&](int k, int i, int j, int e) {
+        LD sum = 0;
+        for (int t = 0; t <= std::min(i + j, e); ++t) sum += E[k][((size_t)i * (lb + 1) + j) * TT + t] * M[k][e][t];
+        return sum;
+      };
+      for (size_t o = 0; o < ops.size(); ++o)
+        for (int ia = 0; ia < na; ++ia) for (int ib = 0; ib < nb; ++ib)
+          m_[o][ia * nb + ib] += cab * mom(0, ax[ia], bx[ib], ops[o][0]) * mom(1, ay[ia], by[ib], ops[o][1]) *
+                                 mom(2, az[ia], bz[ib], ops[o][2]);
     }
+    for (size_t o = 0; o < ops.size(); ++o) scatter_block(A, B, m_[o], out + o * (size_t)N * N, N);
   }
 }
 
@@ -765,6 +828,11 @@ void orc_eri_tensor(void* h, double* eri, int nthreads) {
 // S, T, V column-major N x N.  Z, xyz: nuclear charges and positions (bohr).
 void orc_one_electron(void* h, int natom, const double* Z, const double* xyz, double* S, double* T, double* V) {
   overlap_kinetic_nuclear(*static_cast<Basis*>(h), natom, Z, xyz, S, T, V);
+}
+
+// Multipole integrals of orders 1 .. order (3, 9 or 19 column-major N x N matrices) about C.
+void orc_multipole(void* h, int order, const double* C, double* out) {
+  multipole_ints(*static_cast<Basis*>(h), order, C, out);
 }
 
 // Screening decisions only (the test of aointegrals_contract.h:111-131): number of canonical

@@ -248,3 +248,68 @@ def test_first_oracle_call_is_multithreaded():
     for _ in range(3):
         r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
         assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr
+
+
+AU_PER_DEBYE, ANGSTROM_PER_BOHR = 0.3934303070, 0.5291772083   # tests/rununit.py:312-313
+QUAD_PAIRS = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+OCT_TRIPLES = [(i, j, k) for i in range(3) for j in range(i, 3) for k in range(j, 3)]
+
+
+def _multipoles(P, Mo, mol):
+    """Reference units and conventions (singleslater_properties.h:115-172): -Tr(P r...) plus
+    the nuclear sum, dipole in Debye, quadrupole (plain second moment) in Debye Angstrom,
+    octupole in Debye Angstrom^2."""
+    Z = mol.Z.astype(np.float64)
+    R = mol.coords_bohr
+    el = np.array([np.sum(P * m) for m in Mo])
+    dip = (-el[:3] + Z @ R) / AU_PER_DEBYE
+    quad = np.array([-el[3 + k] + np.sum(Z * R[:, i] * R[:, j]) for k, (i, j) in enumerate(QUAD_PAIRS)])
+    octu = np.array([-el[9 + n] + np.sum(Z * R[:, i] * R[:, j] * R[:, k]) for n, (i, j, k) in enumerate(OCT_TRIPLES)])
+    return dip, quad * ANGSTROM_PER_BOHR / AU_PER_DEBYE, octu * ANGSTROM_PER_BOHR ** 2 / AU_PER_DEBYE
+
+
+@pytest.mark.parametrize("name", ["sto-3g", "cc-pvdz"])
+def test_multipole_translation_and_symmetry(water, name):
+    """x_C = x - C: every moment about C is the binomial expansion of the moments about the
+    origin and the overlap (a wrong shift, index or sign in any order breaks it)."""
+    bs = _bs(name, water)
+    ob = oracle.OracleBasis(bs)
+    S = ob.one_electron(water)[0]
+    M0 = ob.multipole(3)
+    C = np.array([0.3, -0.7, 1.1])
+    MC = ob.multipole(3, C)
+    assert max(np.abs(m - m.T).max() for m in M0) < 1e-14
+    ops = [(1, 0, 0), (0, 1, 0), (0, 0, 1)] + \
+          [tuple(int(a == i) + int(a == j) for a in range(3)) for i, j in QUAD_PAIRS] + \
+          [tuple(int(a == i) + int(a == j) + int(a == k) for a in range(3)) for i, j, k in OCT_TRIPLES]
+    index = {op: n for n, op in enumerate(ops)}
+
+    def about_origin(e):
+        return S if sum(e) == 0 else M0[index[e]]
+
+    for n, e in enumerate(ops):
+        # prod_d (x_d - C_d)^e_d = sum_{f <= e} prod_d C(e_d, f_d) (-C_d)^(e_d - f_d) x_d^f_d
+        ref = np.zeros_like(S)
+        for fx in range(e[0] + 1):
+            for fy in range(e[1] + 1):
+                for fz in range(e[2] + 1):
+                    c = (math.comb(e[0], fx) * (-C[0]) ** (e[0] - fx) * math.comb(e[1], fy) * (-C[1]) ** (e[1] - fy) *
+                         math.comb(e[2], fz) * (-C[2]) ** (e[2] - fz))
+                    ref += c * about_origin((fx, fy, fz))
+        assert np.abs(MC[n] - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), e
+
+
+def test_golden_multipoles(water):
+    """The oracle's multipoles of its own converged RHF densities against the reference's golden
+    dipoles, quadrupoles and octupoles of test0001-0003 (tolerances of tests/rununit.py:39)."""
+    d = json.load(open(os.path.join(GOLD, "energies.json")))
+    tol = d["multipole_tolerance"]
+    for test, rec in d["tests"].items():
+        bs = _bs(rec["basis"], water)
+        r = scf.rhf(water, bs, tau=1e-14)
+        dip, quad, octu = _multipoles(r["P"], oracle.OracleBasis(bs).multipole(3), water)
+        assert np.abs(dip - rec["dipole"]).max() < tol["dipole"], test
+        assert np.abs(quad - rec["quadrupole"]).max() < tol["quadrupole"], test
+        assert np.abs(octu - rec["octupole"]).max() < tol["octupole"], test
+        # far inside the tolerances: the density is right to first order
+        assert np.abs(dip - rec["dipole"]).max() < 5e-6 and np.abs(quad - rec["quadrupole"]).max() < 5e-6
