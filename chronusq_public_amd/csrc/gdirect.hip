@@ -924,10 +924,13 @@ size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int u
 // single-pair groups on (H2O)16/cc-pVDZ, tau = 1e-12 (per-class kernel + screening time): the
 // bound trades fewer K-column flushes (large) against more work items to balance (small).
 // Built-in tile of each class pair: the largest group (uniform pairs sharing a shell) and the
-// union bound (swept pairs per task region).  Measured on (H2O)16/cc-pVDZ with screening in one
-// launch: groups of 2-4 members pay where a swept pair's J slot would otherwise be flushed to
-// HBM once per uniform pair (low angular momentum, thousands of uniform pairs); elsewhere
-// single-pair groups are as fast.
+// union bound (swept pairs per task region), the group size then lowered until the LDS budget
+// holds it.  One molecule (measured on (H2O)16/cc-pVDZ with screening in one launch): groups of
+// 2-4 members pay where a swept pair's J slot would otherwise be flushed to HBM once per uniform
+// pair (low angular momentum, thousands of uniform pairs); elsewhere single-pair groups are as
+// fast.  Several molecules in one context (measured on 256 alkanes and water clusters,
+// 6-31G*, groups of 4096 functions): a uniform pair has few partners inside its small
+// molecule, so large groups fill the wavefronts (serialised kernel time 306 -> 273 ms).
 struct TileDefault { const char* name; int max_group, ub; };
 static const TileDefault kTileDefaults[] = {
     {"ssss", 1, 512}, {"ssps", 4, 256}, {"psps", 4, 512}, {"sspp", 2, 512}, {"ppps", 4, 512},
@@ -935,29 +938,39 @@ static const TileDefault kTileDefaults[] = {
     {"ssdp", 1, 512}, {"dpps", 1, 512}, {"dppp", 1, 1024}, {"dpds", 1, 256}, {"dpdp", 1, 512},
     {"ddss", 1, 256}, {"ddps", 1, 512}, {"ddpp", 1, 512}, {"ddds", 1, 512}, {"dddp", 1, 512},
     {"dddd", 1, 256}};
+static const TileDefault kTileDefaultsMulti[] = {
+    {"ssss", 4, 512}, {"ssps", 16, 512}, {"psps", 16, 1024}, {"sspp", 4, 512}, {"ppps", 16, 1024},
+    {"pppp", 16, 512}, {"ssds", 16, 256}, {"psds", 16, 512}, {"ppds", 8, 256}, {"dsds", 16, 512},
+    {"ssdp", 16, 1024}, {"dpps", 2, 1024}, {"dppp", 16, 256}, {"dpds", 16, 256}, {"dpdp", 16, 256},
+    {"ddss", 2, 1024}, {"ddps", 2, 1024}, {"ddpp", 16, 512}, {"ddds", 16, 256}, {"dddp", 4, 1024},
+    {"dddd", 4, 512}};
 
-const TileDefault* tile_default(const ClassKernels& ck) {
-  for (const auto& t : kTileDefaults)
-    if (std::strcmp(t.name, ck.name) == 0) return &t;
+static_assert(sizeof(kTileDefaults) / sizeof(TileDefault) == CQ_NUM_CLASS_PAIRS, "one tile per class pair");
+static_assert(sizeof(kTileDefaultsMulti) / sizeof(TileDefault) == CQ_NUM_CLASS_PAIRS, "one tile per class pair");
+
+const TileDefault* tile_default(const cq_ctx* c, const ClassKernels& ck) {
+  const TileDefault* tab = c->nmol > 1 ? kTileDefaultsMulti : kTileDefaults;
+  for (int i = 0; i < CQ_NUM_CLASS_PAIRS; ++i)
+    if (std::strcmp(tab[i].name, ck.name) == 0) return &tab[i];
   return nullptr;
 }
 
-int default_union_bound(const ClassKernels& ck, size_t jside) {
-  if (const TileDefault* t = tile_default(ck)) return t->ub;
+int default_union_bound(const cq_ctx* c, const ClassKernels& ck, size_t jside) {
+  if (const TileDefault* t = tile_default(c, ck)) return t->ub;
   return (int)std::min<size_t>(UNION_MAX, std::max<size_t>(64, (16 * 1024) / (jside * sizeof(double)) / 64 * 64));
 }
 
 void class_tile(const cq_ctx* c, int k, int variant, int& mmax, int& ub) {
   const ClassKernels& ck = kClassKernels[k];
   const size_t jside = variant == 2 ? (size_t)ck.nc * ck.nd : (size_t)ck.na * ck.nb;
-  ub = default_union_bound(ck, jside);
+  ub = default_union_bound(c, ck, jside);
   if (c->tile_ub[k] > 0) ub = std::max(64, std::min(c->tile_ub[k], UNION_MAX));
   const size_t budget = c->tile_lds[k] > 0 ? (size_t)c->tile_lds[k] : (size_t)48 * 1024;
   const size_t limit = std::min(budget, c->max_dyn_lds);
   // a union bound whose J slots alone overflow the device's LDS is lowered (multiples of 64)
   while (ub > 64 && group_lds(ck, variant, c->lds_n, 1, ub) > c->max_dyn_lds) ub -= 64;
   mmax = 0;
-  const TileDefault* t = tile_default(ck);
+  const TileDefault* t = tile_default(c, ck);
   const int cap = c->tile_mmax[k] > 0 ? std::min(c->tile_mmax[k], MAX_GROUP) : (t ? t->max_group : 1);
   for (int mm = cap; mm >= 1; --mm)
     if (group_lds(ck, variant, c->lds_n, mm, ub) <= limit) { mmax = mm; break; }
