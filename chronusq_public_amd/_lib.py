@@ -28,7 +28,7 @@ EXPORTED = [
     "cq_num_class_pairs", "cq_class_pair_info", "cq_debug_quartets", "cq_set_profiling",
     "cq_get_kernel_times", "cq_get_screen_times", "cq_set_screen_fusion", "cq_get_work_stats", "cq_one_electron", "cq_one_electron_batch", "cq_multipole", "cq_set_launch_streams", "cq_set_lds_digestion",
     "cq_solid_harmonics", "cq_set_class_digestion", "cq_get_class_digestion",
-    "cq_set_class_stream", "cq_set_task_budget", "cq_get_task_plan", "cq_set_class_tile", "cq_get_class_tile",
+    "cq_set_class_stream", "cq_set_class_chunk_group", "cq_set_task_budget", "cq_get_task_plan", "cq_set_class_tile", "cq_get_class_tile",
     # learned initial guess (include/cqgpu_guess.h)
     "cq_mlp_last_error", "cq_mlp_gemm_bf16", "cq_mlp_relu_backward", "cq_mlp_column_sum", "cq_mlp_to_bf16",
     "cq_mlp_mse", "cq_mlp_adam",
@@ -92,6 +92,7 @@ def load():
     L.cq_set_class_digestion.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.cq_get_class_digestion.argtypes = [vp, ctypes.c_int, ip]
     L.cq_set_class_stream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.cq_set_class_chunk_group.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.cq_set_task_budget.argtypes = [vp, ctypes.c_longlong]
     L.cq_get_task_plan.argtypes = [vp, P(ctypes.c_longlong), ip]
     L.cq_set_class_tile.argtypes = [vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int]

@@ -866,6 +866,8 @@ struct cq_ctx {
   std::vector<int> class_variant = std::vector<int>(CQ_NUM_CLASS_PAIRS, -1);
   // per class pair: the auxiliary stream its kernels go to (-1: round robin in launch order)
   std::vector<int> class_stream = std::vector<int>(CQ_NUM_CLASS_PAIRS, -1);
+  // per class pair: chunks per work item of the group kernels (0: the built-in choice)
+  std::vector<int> class_cpi = std::vector<int>(CQ_NUM_CLASS_PAIRS, 0);
   DevBuf<unsigned long long> d_qcounts, d_nglob;
   DevBuf<unsigned char> d_pair_nblk, d_pair_nublk;
   std::string err;
@@ -931,19 +933,23 @@ size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int u
 // fast.  Several molecules in one context (measured on 256 alkanes and water clusters,
 // 6-31G*, groups of 4096 functions): a uniform pair has few partners inside its small
 // molecule, so large groups fill the wavefronts (serialised kernel time 306 -> 273 ms).
-struct TileDefault { const char* name; int max_group, ub; };
+// cpi: chunks of the split (high angular momentum) class pairs per work item, 0 = all of them.
+// A work item's chunks share its wavefronts and its one LDS flush; more chunks per item means
+// fewer items, which starves the device when a class pair has few task regions.  (H2O)16:
+// (dp|dp) 0.29 -> 0.17 ms at 3, (dd|dd) 0.14 -> 0.08 ms; the batch: G[D] 152 -> 118 ms with all.
+struct TileDefault { const char* name; int max_group, ub, cpi; };
 static const TileDefault kTileDefaults[] = {
-    {"ssss", 1, 512}, {"ssps", 4, 256}, {"psps", 4, 512}, {"sspp", 2, 512}, {"ppps", 4, 512},
-    {"pppp", 1, 256}, {"ssds", 1, 512}, {"psds", 3, 512}, {"ppds", 1, 256}, {"dsds", 1, 256},
-    {"ssdp", 1, 512}, {"dpps", 1, 512}, {"dppp", 1, 1024}, {"dpds", 1, 256}, {"dpdp", 1, 512},
-    {"ddss", 1, 256}, {"ddps", 1, 512}, {"ddpp", 1, 512}, {"ddds", 1, 512}, {"dddp", 1, 512},
-    {"dddd", 1, 256}};
+    {"ssss", 1, 512, 1}, {"ssps", 4, 256, 1}, {"psps", 4, 512, 1}, {"sspp", 2, 512, 1}, {"ppps", 4, 512, 1},
+    {"pppp", 1, 256, 1}, {"ssds", 1, 512, 1}, {"psds", 3, 512, 1}, {"ppds", 1, 256, 1}, {"dsds", 1, 256, 1},
+    {"ssdp", 1, 512, 1}, {"dpps", 1, 512, 1}, {"dppp", 1, 1024, 3}, {"dpds", 1, 256, 1}, {"dpdp", 1, 512, 3},
+    {"ddss", 1, 256, 1}, {"ddps", 1, 512, 1}, {"ddpp", 1, 512, 1}, {"ddds", 1, 512, 1}, {"dddp", 1, 512, 2},
+    {"dddd", 1, 256, 3}};
 static const TileDefault kTileDefaultsMulti[] = {
-    {"ssss", 4, 512}, {"ssps", 16, 512}, {"psps", 16, 1024}, {"sspp", 4, 512}, {"ppps", 16, 1024},
-    {"pppp", 16, 512}, {"ssds", 16, 256}, {"psds", 16, 512}, {"ppds", 8, 256}, {"dsds", 16, 512},
-    {"ssdp", 16, 1024}, {"dpps", 2, 1024}, {"dppp", 16, 256}, {"dpds", 16, 256}, {"dpdp", 16, 256},
-    {"ddss", 2, 1024}, {"ddps", 2, 1024}, {"ddpp", 16, 512}, {"ddds", 16, 256}, {"dddp", 4, 1024},
-    {"dddd", 4, 512}};
+    {"ssss", 4, 512, 0}, {"ssps", 16, 512, 0}, {"psps", 16, 1024, 0}, {"sspp", 4, 512, 0}, {"ppps", 16, 1024, 0},
+    {"pppp", 16, 512, 0}, {"ssds", 16, 256, 0}, {"psds", 16, 512, 0}, {"ppds", 8, 256, 0}, {"dsds", 16, 512, 0},
+    {"ssdp", 16, 1024, 0}, {"dpps", 2, 1024, 0}, {"dppp", 16, 256, 0}, {"dpds", 16, 256, 0}, {"dpdp", 16, 256, 0},
+    {"ddss", 2, 1024, 0}, {"ddps", 2, 1024, 0}, {"ddpp", 16, 512, 0}, {"ddds", 16, 256, 0}, {"dddp", 4, 1024, 0},
+    {"dddd", 4, 512, 0}};
 
 static_assert(sizeof(kTileDefaults) / sizeof(TileDefault) == CQ_NUM_CLASS_PAIRS, "one tile per class pair");
 static_assert(sizeof(kTileDefaultsMulti) / sizeof(TileDefault) == CQ_NUM_CLASS_PAIRS, "one tile per class pair");
@@ -958,6 +964,17 @@ const TileDefault* tile_default(const cq_ctx* c, const ClassKernels& ck) {
 int default_union_bound(const cq_ctx* c, const ClassKernels& ck, size_t jside) {
   if (const TileDefault* t = tile_default(c, ck)) return t->ub;
   return (int)std::min<size_t>(UNION_MAX, std::max<size_t>(64, (16 * 1024) / (jside * sizeof(double)) / 64 * 64));
+}
+
+// Chunks per work item of class pair k: the caller's choice, else the built-in table's; always
+// a divisor of the chunk count.
+int class_chunks_per_item(const cq_ctx* c, int k) {
+  const int nch = kClassKernels[k].nchunk;
+  if (c->class_cpi[k] > 0) return c->class_cpi[k];
+  const TileDefault* t = tile_default(c, kClassKernels[k]);
+  int d = (!t || t->cpi == 0) ? nch : std::min(t->cpi, nch);
+  while (d > 1 && nch % d != 0) --d;
+  return std::max(1, d);
 }
 
 void class_tile(const cq_ctx* c, int k, int variant, int& mmax, int& ub) {
@@ -1669,6 +1686,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
         throw std::runtime_error(std::string("class pair ") + ck.name + ": the planned tile does not fit the device's LDS");
       GroupKernel kern = ck.grp[fp32 ? 1 : 0][kv];
       const int wgs = kv == 2 ? WG : ck.seg_wg;
+      const int cpi = class_chunks_per_item(c, k);
       unsigned* wq = c->d_work_q.p + ((size_t)s0 * kNumClassPairs + k) * WORK_QUEUES;
       // batches of groups, in order on this class pair's stream: a class pair in one batch
       // screens once and keeps its lists for every slot pass; a batched one clears its counters
@@ -1701,11 +1719,12 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
           HIPCHK(hipEventRecord(e0, sk));
         }
         // (the overflow regions are known on the device only: the grid covers the segment capacity)
-        const int sgrid = (int)std::min<long long>((long long)persistent_grid() * WG / wgs, m.seg_cap * ck.nchunk);
+        const int sgrid = (int)std::min<long long>((long long)persistent_grid() * WG / wgs,
+                                                   m.seg_cap * (ck.nchunk / cpi));
         hipLaunchKernelGGL(kern, dim3(std::max(1, sgrid)), dim3(wgs), lds_bytes, sk, c->d_tasks.p + m.task_off,
                            c->d_segs.p + m.seg_off, nslots, c->d_ovf.p + k, wq, c->d_pairs.p, c->d_prims.p,
                            c->d_boys.p, jk,
-                           c->d_groups[tb].p, c->d_uorder.p, m.mmax, m.ub);
+                           c->d_groups[tb].p, c->d_uorder.p, m.mmax, m.ub, cpi);
         HIPCHK(hipGetLastError());
         if (profiling) {
           HIPCHK(hipEventRecord(e1, sk));
@@ -2100,6 +2119,20 @@ int cq_set_class_stream(cq_ctx_t* c, int k, int stream) {
     if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");
     if (stream < -1 || stream >= cq_ctx::kAux) throw std::invalid_argument("stream index must be -1 .. 3");
     c->class_stream[k] = stream;
+  });
+}
+
+// At most cpi chunks of a class pair's ket (or bra) components per work item of its group
+// kernels (the largest divisor of the class pair's chunk count not above cpi); 0 for the
+// built-in choice.
+int cq_set_class_chunk_group(cq_ctx_t* c, int k, int cpi) {
+  return guarded(c, [&] {
+    if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");
+    if (cpi < 0) throw std::invalid_argument("chunks per work item must be >= 0");
+    const int nch = kClassKernels[k].nchunk;
+    int d = std::min(cpi, nch);
+    while (d > 1 && nch % d != 0) --d;
+    c->class_cpi[k] = cpi == 0 ? 0 : d;
   });
 }
 

@@ -125,9 +125,10 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
   return code < 0 ? (long)(~code) + (long)j * N + i : (long)code + (long)i * N + j;
 }
 
-// Group kernels.  Work items are (task region, chunk) pairs from an atomic queue; each workgroup
-// runs the region's members one after the other (the member wavefront-uniform, one quartet per
-// lane), then flushes its LDS accumulators to HBM:
+// Group kernels.  Work items are (task region, group of cpi chunks) pairs from an atomic queue
+// (cpi divides the class pair's chunk count); each workgroup spreads the item's (chunk, block of
+// 64 entries) units over its wavefronts -- every member's entries in one dense run, the member
+// wavefront-uniform, one quartet per lane -- then flushes its LDS accumulators to HBM once:
 //  * ket-keyed (eri_grp, DIG 0 or 1): K columns of the shared c shell and of every member's d
 //    shell (DIG 1; with DIG 0 K goes to HBM atomics directly), J(c, d_m) of every member, and J of
 //    every union bra;
@@ -143,14 +144,15 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
       int nres, const unsigned* __restrict__ ovf, unsigned* __restrict__ work,                \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                 \
       const double* __restrict__ boys, JKArgs jk, const GroupRec* __restrict__ groups,        \
-      const int* __restrict__ uorder, int mmax, int ub) {                                     \
+      const int* __restrict__ uorder, int mmax, int ub, int cpi) {                            \
     constexpr int NCH = kChunks_##NAME, NC = kNC_##NAME, ND = kND_##NAME;                     \
     constexpr int NA = kNA_##NAME, NB = kNB_##NAME, NCD = NC * ND, NAB = NA * NB;             \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                      \
     __shared__ int s_item;                                                                    \
     __shared__ int s_dcol[MAX_GROUP], s_moff[MAX_GROUP];                                      \
     extern __shared__ double dyn_lds[];                                                       \
-    const long nitems = ((long)nres + *ovf) * NCH;                                            \
+    const int nit = NCH / cpi;   /* items per task region: groups of cpi chunks */            \
+    const long nitems = ((long)nres + *ovf) * nit;                                            \
     if (nitems == 0) return;                                                                  \
     stage_boys(boys, LTOT, boys_lds);                                                         \
     const int N = jk.ncart, LN = jk.lds_n;                                                    \
@@ -160,13 +162,13 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
     double* jsl = jcdL + mmax * NCD;                                                          \
     int* spair = (int*)(jsl + ub * NAB);                                                      \
     for (int k = threadIdx.x; k < ncolmax + mmax * NCD + ub * NAB; k += WGS) dyn_lds[k] = 0.0; \
-    const int wofs = threadIdx.x & ~63;                                                       \
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;                                \
     while (true) {                                                                            \
       const long item = next_item(work, &s_item);                                             \
       if (item >= nitems) break;                                                              \
-      const Segment sg = segs[item / NCH];                                                    \
+      const Segment sg = segs[item / nit];                                                    \
       if (sg.count == 0) continue;   /* a slot its group did not fill */                      \
-      const int chunk = (int)(item % NCH);                                                    \
+      const int ch0 = (int)(item % nit) * cpi;                                                \
       const GroupRec gr = groups[sg.ket];                                                     \
       const int n = gr.n;                                                                     \
       const unsigned kblk = __builtin_amdgcn_readfirstlane((unsigned)sg.blk);                 \
@@ -177,13 +179,14 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
         s_moff[threadIdx.x] = (int)reg[1 + threadIdx.x].y;                                    \
       }                                                                                       \
       __syncthreads();                                                                        \
-      /* every member's list, one dense run: waterfall over the members of a wavefront */     \
-      const int e0 = 1 + n, E = sg.count - e0;                                                \
-      for (int t0 = 0; t0 < E; t0 += WGS) {                                                   \
-        if (t0 + wofs >= E) continue;                                                         \
-        const int t = t0 + threadIdx.x;                                                       \
+      /* every member's list, one dense run: waterfall over the members of a wavefront;      \
+         units (chunk, 64-entry block), chunk-major, over the workgroup's wavefronts */         \
+      const int e0 = 1 + n, E = sg.count - e0, nblk = (E + 63) >> 6;                          \
+      for (int u = wid; u < nblk * cpi; u += WGS / 64) {                                      \
+        const int chunk = ch0 + u / nblk, b0 = (u % nblk) << 6;                               \
+        const int t = b0 + lane;                                                              \
         const bool active = t < E;                                                            \
-        const int pos = e0 + (active ? t : t0 + wofs);                                        \
+        const int pos = e0 + (active ? t : b0);                                               \
         const uint2 e = reg[pos];                                                             \
         int mi = 0;                                                                           \
         for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                      \
@@ -269,14 +272,15 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
       int nres, const unsigned* __restrict__ ovf, unsigned* __restrict__ work,                  \
       const PairInfo* __restrict__ pairs, const PrimPair* __restrict__ prims,                   \
       const double* __restrict__ boys, JKArgs jk, const GroupRec* __restrict__ groups,          \
-      const int* __restrict__ uorder, int mmax, int ub) {                                       \
+      const int* __restrict__ uorder, int mmax, int ub, int cpi) {                              \
     constexpr int NCH = kChunks_##NAME, NC = kNC_##NAME, ND = kND_##NAME;                       \
     constexpr int NA = kNA_##NAME, NB = kNB_##NAME, NCD = NC * ND, NAB = NA * NB;               \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                        \
     __shared__ int s_item;                                                                      \
     __shared__ int s_brow[MAX_GROUP], s_moff[MAX_GROUP];                                        \
     extern __shared__ double dyn_lds[];                                                         \
-    const long nitems = ((long)nres + *ovf) * NCH;                                              \
+    const int nit = NCH / cpi;   /* items per task region: groups of cpi chunks */              \
+    const long nitems = ((long)nres + *ovf) * nit;                                              \
     if (nitems == 0) return;                                                                    \
     stage_boys(boys, LTOT, boys_lds);                                                           \
     const int N = jk.ncart, LN = jk.lds_n;                                                      \
@@ -286,13 +290,13 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
     double* jsl = jabL + mmax * NAB;                                                            \
     int* spair = (int*)(jsl + ub * NCD);                                                        \
     for (int k = threadIdx.x; k < nrowmax + mmax * NAB + ub * NCD; k += WG) dyn_lds[k] = 0.0;   \
-    const int wofs = threadIdx.x & ~63;                                                         \
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;                                  \
     while (true) {                                                                              \
       const long item = next_item(work, &s_item);                                               \
       if (item >= nitems) break;                                                                \
-      const Segment sg = segs[item / NCH];                                                      \
+      const Segment sg = segs[item / nit];                                                      \
       if (sg.count == 0) continue;   /* a slot its group did not fill */                        \
-      const int chunk = (int)(item % NCH);                                                      \
+      const int ch0 = (int)(item % nit) * cpi;                                                  \
       const GroupRec gr = groups[sg.ket];                                                       \
       const int n = gr.n;                                                                       \
       const unsigned bblk = __builtin_amdgcn_readfirstlane((unsigned)sg.blk);                   \
@@ -303,12 +307,12 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
         s_moff[threadIdx.x] = (int)reg[1 + threadIdx.x].y;                                      \
       }                                                                                         \
       __syncthreads();                                                                          \
-      const int e0 = 1 + n, E = sg.count - e0;                                                  \
-      for (int t0 = 0; t0 < E; t0 += WG) {                                                      \
-        if (t0 + wofs >= E) continue;                                                           \
-        const int t = t0 + threadIdx.x;                                                         \
+      const int e0 = 1 + n, E = sg.count - e0, nblk = (E + 63) >> 6;                            \
+      for (int u = wid; u < nblk * cpi; u += WG / 64) {                                         \
+        const int chunk = ch0 + u / nblk, b0 = (u % nblk) << 6;                                 \
+        const int t = b0 + lane;                                                                \
         const bool active = t < E;                                                              \
-        const int pos = e0 + (active ? t : t0 + wofs);                                          \
+        const int pos = e0 + (active ? t : b0);                                                 \
         const uint2 e = reg[pos];                                                               \
         int mi = 0;                                                                             \
         for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                        \
@@ -362,7 +366,7 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
 typedef void (*BlockKernel)(const uint2*, long, const PairInfo*, const PrimPair*, const double*, double*, int,
                             const int*);
 typedef void (*GroupKernel)(const uint2*, const Segment*, int, const unsigned*, unsigned*, const PairInfo*,
-                            const PrimPair*, const double*, JKArgs, const GroupRec*, const int*, int, int);
+                            const PrimPair*, const double*, JKArgs, const GroupRec*, const int*, int, int, int);
 
 struct ClassKernels {
   int i, j, bi, ki, L, nchunk;

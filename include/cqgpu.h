@@ -157,6 +157,13 @@ int cq_set_lds_digestion(cq_ctx_t* ctx, int mode);
  * a tuner pick the fastest variant per class pair from cq_get_kernel_times; results agree to
  * rounding in every variant. */
 int cq_set_class_digestion(cq_ctx_t* ctx, int k, int mode);
+/* Chunks per work item of class pair k's group kernels (no reference counterpart): the ket (or
+ * bra) Cartesian components of the high-angular-momentum class pairs are split into up to 36
+ * chunks by register budget; a work item takes a task region and several of its chunks, spread
+ * over the workgroup's wavefronts, and flushes its LDS accumulators once.  cpi >= 1: at most cpi
+ * chunks per item (the largest divisor of the chunk count not above cpi); 0 restores the
+ * built-in choice. */
+int cq_set_class_chunk_group(cq_ctx_t* ctx, int k, int cpi);
 int cq_get_class_digestion(const cq_ctx_t* ctx, int k, int* mode);
 
 /* Auxiliary stream (0..3) the kernels of class pair k are launched on, -1 (default) for

@@ -387,3 +387,28 @@ def test_screen_fusion_matches_per_class_screening():
         ao.setPrecision("fp64")
     assert res[0][2] == res[1][2] and res[0][2][1] > 0
     assert np.abs(res[0][0] - res[1][0]).max() < 1e-12 and np.abs(res[0][1] - res[1][1]).max() < 1e-12
+
+
+@pytest.mark.parametrize("cpi", [1, 2, 3, 36])
+def test_chunks_per_work_item_match_oracle(cpi):
+    """The split class pairs ((dd|dd) in 36 chunks, (dp|dp) and (dd|dp) in 18, ...) with 1, 2, 3
+    or all of a task region's chunks per work item, spread over the workgroup's wavefronts and
+    flushed once, in every digestion variant: J and K against the oracle with identical
+    surviving-quartet counts ((H2O)8 / cc-pVDZ)."""
+    from chronusq_public_amd import _lib
+    L = _lib.load()
+    mol = M.water_cluster(8, seed=11, box_ang=M.scaled_box(8))
+    bs, ao, ob = _setup(mol, "cc-pvdz")
+    X = _sym_random(bs.nbasis, 23)
+    Qg = ao.computeSchwartz()
+    (Jo, Ko), n = _oracle_jk(ob, Qg, [X, X], [0, 1], [1.0, -0.5], 1e-12)
+    for k in range(L.cq_num_class_pairs()):
+        _lib.check(L.cq_set_class_chunk_group(ao._h, k, cpi), ao._h)
+    for lds_mode in (2, 1, 0):
+        print(f"[chunks] cpi={cpi} lds_mode={lds_mode}", flush=True)
+        _lib.check(L.cq_set_lds_digestion(ao._h, lds_mode), ao._h)
+        J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
+        assert np.abs(J - Jo).max() < TOL and np.abs(K - Ko).max() < TOL, (cpi, lds_mode)
+        assert sum(ao.quartet_counts().values()) == n
+    with pytest.raises(_lib.CQError):
+        _lib.check(L.cq_set_class_chunk_group(ao._h, 0, -1), ao._h)
