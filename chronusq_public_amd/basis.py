@@ -218,19 +218,38 @@ class BasisSet:
         return l, pure, nprim, alpha, coef, origin
 
 
+_REFERENCE_SHELLS: Dict[tuple, Dict[int, list]] = {}
+
+
+def _reference_shells(name: str, force_cart: bool) -> Dict[int, list]:
+    """Per element: (l, pure, alpha, normalised coefficients) of its reference shells, parsed
+    and normalised once per process (read-only arrays, shared by every atom built from them)."""
+    key = (basis_path(name), force_cart)
+    if key not in _REFERENCE_SHELLS:
+        ref = parse_gbs(key[0], force_cart=force_cart)
+        out = {}
+        for z, shells in ref.items():
+            lst = []
+            for rs in shells:
+                alpha = np.asarray(rs.alpha, dtype=np.float64)
+                coef = normalize_coefficients(rs.l, rs.alpha, rs.coef)
+                alpha.setflags(write=False)
+                coef.setflags(write=False)
+                lst.append((rs.l, rs.pure, alpha, coef))
+            out[z] = lst
+        _REFERENCE_SHELLS[key] = out
+    return _REFERENCE_SHELLS[key]
+
+
 def build_basis(name: str, atomic_numbers: Sequence[int], coords_bohr: np.ndarray,
                 force_cart: bool = False) -> BasisSet:
     """constructLocal: one copy of the element's reference shells per atom, in atom order."""
-    ref = parse_gbs(basis_path(name), force_cart=force_cart)
+    ref = _reference_shells(name, force_cart)
     coords_bohr = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
     bs = BasisSet(name)
     for iat, z in enumerate(atomic_numbers):
         if z not in ref:
             raise ValueError(f"Atomic Number {z} not found in current Basis Set")
-        for rs in ref[z]:
-            bs.shells.append(Shell(
-                l=rs.l, pure=rs.pure,
-                alpha=np.asarray(rs.alpha, dtype=np.float64),
-                coef=normalize_coefficients(rs.l, rs.alpha, rs.coef),
-                origin=coords_bohr[iat].copy(), atom=iat))
+        for l, pure, alpha, coef in ref[z]:
+            bs.shells.append(Shell(l=l, pure=pure, alpha=alpha, coef=coef, origin=coords_bohr[iat].copy(), atom=iat))
     return bs

@@ -45,12 +45,12 @@ class Molecule:
         return int(self.Z.sum()) - self.charge
 
     def nuclear_repulsion(self) -> float:
-        e = 0.0
-        for i in range(self.natom):
-            for j in range(i + 1, self.natom):
-                r = self.coords_bohr[i] - self.coords_bohr[j]
-                e += self.Z[i] * self.Z[j] / math.sqrt(float(r @ r))
-        return e
+        """sum_{i<j} Z_i Z_j / |R_i - R_j| (bohr, hartree)."""
+        R = np.asarray(self.coords_bohr, dtype=np.float64)
+        Z = np.asarray(self.Z, dtype=np.float64)
+        i, j = np.triu_indices(self.natom, k=1)
+        d = np.sqrt(((R[i] - R[j]) ** 2).sum(axis=1))
+        return float(np.sum(Z[i] * Z[j] / d))
 
     def symbols(self) -> List[str]:
         return [ELEMENTS[z] for z in self.Z]
