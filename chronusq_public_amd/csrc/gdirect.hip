@@ -2174,9 +2174,9 @@ int cq_set_task_budget(cq_ctx_t* c, long long entries) {
 }
 
 // Tile of class pair k's group kernels: the dynamic LDS budget in bytes (0: the default 48 KiB),
-// the largest group (members of one work item, 0: as many as the budget allows, at most 16) and
-// the union bound of a task region (swept pairs whose J collects in LDS slots, 0: about 16 KiB of
-// slots).  Waits for the context's work and re-plans the task lists.
+// the largest group (members of one work item, at most 16; 0: the built-in table's) and the
+// union bound of a task region (swept pairs whose J collects in LDS slots; 0: the built-in
+// table's).  Waits for the context's work and re-plans the task lists.
 int cq_set_class_tile(cq_ctx_t* c, int k, long long lds_bytes, int max_group, int union_bound) {
   return guarded(c, [&] {
     if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");

@@ -192,9 +192,10 @@ int cq_get_task_plan(const cq_ctx_t* ctx, long long* entries, int* max_batches);
  * swept pairs that survive the screen against any of them; the J of each swept pair collects in
  * an LDS slot across the group (one HBM update per item instead of one per quartet).
  * lds_bytes: dynamic LDS budget of the kernel (0: 48 KiB); max_group: cap on the group size,
- * 1..16, within the budget (0: the default, single pairs); union_bound: swept pairs per task
- * region, 64..1024 (0: the class pair's default).  Waits for the context's work and re-plans.  Results agree to rounding for
- * every tile. */
+ * 1..16, within the budget (0: the class pair's built-in cap, which differs for one-molecule
+ * and several-molecule contexts); union_bound: swept pairs per task region, 64..1024 (0: the
+ * class pair's built-in bound).  Waits for the context's work and re-plans.  Results agree to
+ * rounding for every tile. */
 int cq_set_class_tile(cq_ctx_t* ctx, int k, long long lds_bytes, int max_group, int union_bound);
 /* The planned tile for digestion variant 0/1 (ket-keyed) or 2 (bra-keyed): largest group, union
  * bound, number of groups, dynamic LDS bytes of the kernel. */
