@@ -149,7 +149,7 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
     constexpr int NA = kNA_##NAME, NB = kNB_##NAME, NCD = NC * ND, NAB = NA * NB;             \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                      \
     __shared__ int s_item;                                                                    \
-    __shared__ int s_dcol[MAX_GROUP], s_moff[MAX_GROUP];                                      \
+    __shared__ int s_dcol[MAX_GROUP], s_moff[MAX_GROUP], s_mcnt[MAX_GROUP];                   \
     extern __shared__ double dyn_lds[];                                                       \
     const int nit = NCH / cpi;   /* items per task region: groups of cpi chunks */            \
     const long nitems = ((long)nres + *ovf) * nit;                                            \
@@ -177,34 +177,28 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
       if (threadIdx.x < n) {                                                                  \
         s_dcol[threadIdx.x] = pairs[uorder[gr.first + threadIdx.x]].b0;                       \
         s_moff[threadIdx.x] = (int)reg[1 + threadIdx.x].y;                                    \
+        s_mcnt[threadIdx.x] = (int)reg[1 + threadIdx.x].x;                                    \
       }                                                                                       \
       __syncthreads();                                                                        \
-      /* every member's list, one dense run: waterfall over the members of a wavefront;      \
-         units (chunk, 64-entry block), chunk-major, over the workgroup's wavefronts */         \
-      const int e0 = 1 + n, E = sg.count - e0, nblk = (E + 63) >> 6;                          \
-      for (int u = wid; u < nblk * cpi; u += WGS / 64) {                                      \
-        const int chunk = ch0 + u / nblk, b0 = (u % nblk) << 6;                               \
-        const int t = b0 + lane;                                                              \
-        const bool active = t < E;                                                            \
-        const int pos = e0 + (active ? t : b0);                                               \
-        const uint2 e = reg[pos];                                                             \
-        int mi = 0;                                                                           \
-        for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                      \
+      /* units (chunk, member, 64-entry block of the member's list), chunk-major, over the    \
+         workgroup's wavefronts: the member is wavefront-uniform */                           \
+      int nbt = 0;                                                                            \
+      for (int mm = 0; mm < n; ++mm) nbt += (s_mcnt[mm] + 63) >> 6;                           \
+      for (int u = wid; u < nbt * cpi; u += WGS / 64) {                                       \
+        const int chunk = ch0 + u / nbt;                                                      \
+        int v = u % nbt, mu = 0;                                                              \
+        while (v >= ((s_mcnt[mu] + 63) >> 6)) { v -= (s_mcnt[mu] + 63) >> 6; ++mu; }          \
+        mu = __builtin_amdgcn_readfirstlane(mu);                                              \
+        const int b0 = v << 6, t = b0 + lane;                                                 \
+        const bool active = t < s_mcnt[mu];                                                   \
+        const uint2 e = reg[s_moff[mu] + (active ? t : b0)];                                  \
         const PairInfo B = pairs[pair_of(e.x)];                                               \
         if (active) spair[e.y] = jcode(B, N);                                                 \
-        bool pending = active;                                                                \
-        while (true) {                                                                        \
-          const unsigned long long pm = __ballot(pending);                                    \
-          if (pm == 0ull) break;                                                              \
-          const int mu = __builtin_amdgcn_readfirstlane(__shfl(mi, __ffsll((long long)pm) - 1)); \
-          const bool mine = pending && mi == mu;                                              \
-          const unsigned kk = __builtin_amdgcn_readfirstlane((unsigned)uorder[gr.first + mu]); \
-          QuartetArgs q;                                                                      \
-          make_quartet_uniform_ket(B, e.x, kk, kblk, mine, pairs, prims, q);                  \
-          quartet_jk_##NAME<1, DIG, R>(q, jk, boys_lds, chunk, colK, colK + (NC + mu * ND) * LN, \
-                                       jcdL + mu * NCD, jsl + (long)e.y * NAB);                \
-          pending = pending && !mine;                                                         \
-        }                                                                                     \
+        const unsigned kk = __builtin_amdgcn_readfirstlane((unsigned)uorder[gr.first + mu]);  \
+        QuartetArgs q;                                                                        \
+        make_quartet_uniform_ket(B, e.x, kk, kblk, active, pairs, prims, q);                  \
+        quartet_jk_##NAME<1, DIG, R>(q, jk, boys_lds, chunk, colK, colK + (NC + mu * ND) * LN, \
+                                     jcdL + mu * NCD, jsl + (long)e.y * NAB);                  \
       }                                                                                       \
       __syncthreads();                                                                        \
       const PairInfo K0 = pairs[uorder[gr.first]];                                            \
@@ -277,7 +271,7 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
     constexpr int NA = kNA_##NAME, NB = kNB_##NAME, NCD = NC * ND, NAB = NA * NB;               \
     __shared__ double boys_lds[BOYS_NGRID * BOYS_LDS_W];                                        \
     __shared__ int s_item;                                                                      \
-    __shared__ int s_brow[MAX_GROUP], s_moff[MAX_GROUP];                                        \
+    __shared__ int s_brow[MAX_GROUP], s_moff[MAX_GROUP], s_mcnt[MAX_GROUP];                     \
     extern __shared__ double dyn_lds[];                                                         \
     const int nit = NCH / cpi;   /* items per task region: groups of cpi chunks */              \
     const long nitems = ((long)nres + *ovf) * nit;                                              \
@@ -305,32 +299,26 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
       if (threadIdx.x < n) {                                                                    \
         s_brow[threadIdx.x] = pairs[uorder[gr.first + threadIdx.x]].b0;                         \
         s_moff[threadIdx.x] = (int)reg[1 + threadIdx.x].y;                                      \
+        s_mcnt[threadIdx.x] = (int)reg[1 + threadIdx.x].x;                                      \
       }                                                                                         \
       __syncthreads();                                                                          \
-      const int e0 = 1 + n, E = sg.count - e0, nblk = (E + 63) >> 6;                            \
-      for (int u = wid; u < nblk * cpi; u += WG / 64) {                                         \
-        const int chunk = ch0 + u / nblk, b0 = (u % nblk) << 6;                                 \
-        const int t = b0 + lane;                                                                \
-        const bool active = t < E;                                                              \
-        const int pos = e0 + (active ? t : b0);                                                 \
-        const uint2 e = reg[pos];                                                               \
-        int mi = 0;                                                                             \
-        for (int mm = 1; mm < n; ++mm) mi = pos >= s_moff[mm] ? mm : mi;                        \
+      int nbt = 0;                                                                              \
+      for (int mm = 0; mm < n; ++mm) nbt += (s_mcnt[mm] + 63) >> 6;                             \
+      for (int u = wid; u < nbt * cpi; u += WG / 64) {                                          \
+        const int chunk = ch0 + u / nbt;                                                        \
+        int v = u % nbt, mu = 0;                                                                \
+        while (v >= ((s_mcnt[mu] + 63) >> 6)) { v -= (s_mcnt[mu] + 63) >> 6; ++mu; }            \
+        mu = __builtin_amdgcn_readfirstlane(mu);                                                \
+        const int b0 = v << 6, t = b0 + lane;                                                   \
+        const bool active = t < s_mcnt[mu];                                                     \
+        const uint2 e = reg[s_moff[mu] + (active ? t : b0)];                                    \
         const PairInfo K = pairs[pair_of(e.x)];                                                 \
         if (active) spair[e.y] = jcode(K, N);                                                   \
-        bool pending = active;                                                                  \
-        while (true) {                                                                          \
-          const unsigned long long pm = __ballot(pending);                                      \
-          if (pm == 0ull) break;                                                                \
-          const int mu = __builtin_amdgcn_readfirstlane(__shfl(mi, __ffsll((long long)pm) - 1)); \
-          const bool mine = pending && mi == mu;                                                \
-          const unsigned bb = __builtin_amdgcn_readfirstlane((unsigned)uorder[gr.first + mu]);  \
-          QuartetArgs q;                                                                        \
-          make_quartet_uniform_bra(K, e.x, bb, bblk, mine, pairs, prims, q);                    \
-          quartet_jk_##NAME<1, 2, R>(q, jk, boys_lds, chunk, rowK, rowK + (NA + mu * NB) * LN,   \
-                                     jabL + mu * NAB, jsl + (long)e.y * NCD);                   \
-          pending = pending && !mine;                                                           \
-        }                                                                                       \
+        const unsigned bb = __builtin_amdgcn_readfirstlane((unsigned)uorder[gr.first + mu]);    \
+        QuartetArgs q;                                                                          \
+        make_quartet_uniform_bra(K, e.x, bb, bblk, active, pairs, prims, q);                    \
+        quartet_jk_##NAME<1, 2, R>(q, jk, boys_lds, chunk, rowK, rowK + (NA + mu * NB) * LN,     \
+                                   jabL + mu * NAB, jsl + (long)e.y * NCD);                     \
       }                                                                                         \
       __syncthreads();                                                                          \
       const PairInfo A0 = pairs[uorder[gr.first]];                                              \
