@@ -927,10 +927,10 @@ size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int u
 // bound trades fewer K-column flushes (large) against more work items to balance (small).
 // Built-in tile of each class pair: the largest group (uniform pairs sharing a shell) and the
 // union bound (swept pairs per task region), the group size then lowered until the LDS budget
-// holds it.  One molecule (measured on (H2O)16/cc-pVDZ with screening in one launch): groups of
-// 2-4 members pay where a swept pair's J slot would otherwise be flushed to HBM once per uniform
-// pair (low angular momentum, thousands of uniform pairs); elsewhere single-pair groups are as
-// fast.  Several molecules in one context (measured on 256 alkanes and water clusters,
+// holds it.  One molecule (measured on (H2O)16/cc-pVDZ with screening in one launch and
+// per-member wavefront units): groups of 3-8 members pay where a swept pair's J slot would
+// otherwise be flushed to HBM once per uniform pair ((ps|ps) -10 %, (dp|pp) -41 %); elsewhere
+// single-pair groups are as fast.  Several molecules in one context (measured on 256 alkanes and water clusters,
 // 6-31G*, groups of 4096 functions): a uniform pair has few partners inside its small
 // molecule, so large groups fill the wavefronts (serialised kernel time 306 -> 273 ms).
 // cpi: chunks of the split (high angular momentum) class pairs per work item, 0 = all of them.
@@ -939,9 +939,9 @@ size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int u
 // (dp|dp) 0.29 -> 0.17 ms at 3, (dd|dd) 0.14 -> 0.08 ms; the batch: G[D] 152 -> 118 ms with all.
 struct TileDefault { const char* name; int max_group, ub, cpi; };
 static const TileDefault kTileDefaults[] = {
-    {"ssss", 1, 512, 1}, {"ssps", 4, 256, 1}, {"psps", 4, 512, 1}, {"sspp", 2, 512, 1}, {"ppps", 4, 512, 1},
-    {"pppp", 1, 256, 1}, {"ssds", 1, 512, 1}, {"psds", 3, 512, 1}, {"ppds", 1, 256, 1}, {"dsds", 1, 256, 1},
-    {"ssdp", 1, 512, 1}, {"dpps", 1, 512, 1}, {"dppp", 1, 1024, 3}, {"dpds", 1, 256, 1}, {"dpdp", 1, 512, 3},
+    {"ssss", 4, 256, 1}, {"ssps", 4, 256, 1}, {"psps", 8, 256, 1}, {"sspp", 2, 512, 1}, {"ppps", 4, 512, 1},
+    {"pppp", 3, 256, 1}, {"ssds", 1, 512, 1}, {"psds", 3, 512, 1}, {"ppds", 1, 256, 1}, {"dsds", 1, 256, 1},
+    {"ssdp", 1, 512, 1}, {"dpps", 1, 256, 1}, {"dppp", 8, 256, 3}, {"dpds", 1, 256, 1}, {"dpdp", 1, 512, 3},
     {"ddss", 1, 256, 1}, {"ddps", 1, 512, 1}, {"ddpp", 1, 512, 1}, {"ddds", 1, 512, 1}, {"dddp", 1, 512, 2},
     {"dddd", 1, 256, 3}};
 static const TileDefault kTileDefaultsMulti[] = {
