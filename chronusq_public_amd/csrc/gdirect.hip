@@ -204,6 +204,10 @@ __global__ void __launch_bounds__(WG) screen_kernel(
     mloc[wid][lane] = K - uni_start;
   }
   wave_sync_lds();
+  // the shell every member shares (members are pairs (a, b_m) with one shell a in common):
+  // its norms against a swept pair are read once per swept pair, not once per member
+  int ash = ms1[wid][0];
+  if (n >= 2 && ms1[wid][0] != ms1[wid][1] && ms1[wid][0] != ms2[wid][1]) ash = ms2[wid][0];
   double Qkmax = 0.0;
   int lmin = 0x7fffffff;
   for (int mm = 0; mm < n; ++mm) {
@@ -310,6 +314,12 @@ __global__ void __launch_bounds__(WG) screen_kernel(
         mask[r] = 0u;
         if (i >= grp_end) continue;
         const long canonP = (long)sp[r].x * (sp[r].x + 1) / 2 + sp[r].y;
+        // symmetric norms: N(P) and N(a, p) for the shared shell a, the same for every member
+        double nbase = 0.0;
+        if (symN) {
+          const double* ra = Nmax + (long)ash * nshell;
+          nbase = fmax(Np[r], fmax(ra[sp[r].x], ra[sp[r].y]));
+        }
         for (int mm = 0; mm < n; ++mm) {
           const double Qk = mQ[wid][mm];
           if (nglob * Qk * Qi0 * (1.0 + 1e-12) < tau) continue;   // no swept pair from here passes mm
@@ -320,9 +330,9 @@ __global__ void __launch_bounds__(WG) screen_kernel(
           const double Q12 = pfirst ? Qp[r] : Qk, Q34 = pfirst ? Qk : Qp[r];
           double mx;
           if (symN) {
-            const double* rx = Nmax + (long)kx * nshell;
-            const double* ry = Nmax + (long)ky * nshell;
-            mx = fmax(fmax(Np[r], mN[wid][mm]), fmax(fmax(rx[sp[r].x], rx[sp[r].y]), fmax(ry[sp[r].x], ry[sp[r].y])));
+            // the six norms: N(P), N(K), and N(k, p) for k in {a, b_m}, p in P (N symmetric)
+            const double* ro = Nmax + (long)(kx == ash ? ky : kx) * nshell;
+            mx = fmax(fmax(nbase, mN[wid][mm]), fmax(ro[sp[r].x], ro[sp[r].y]));
           } else {
             int s1, s2, s3, s4;
             if (pfirst) { s1 = sp[r].x; s2 = sp[r].y; s3 = kx; s4 = ky; }
