@@ -62,14 +62,16 @@ struct JKArgs {
   const int* nonsym;             // device flags: density slot is not symmetric
 };
 
-// 1/sqrt(x) for x > 0: hardware estimate (about 2^-23 relative) + two Newton steps.
+// 1/sqrt(x) for x > 0: hardware estimate y0 (about 2^-23 relative), then one third-order
+// correction y = y0 (1 + e/2 + 3e^2/8), e = 1 - x y0^2 (the binomial series of (1 - e)^-1/2;
+// truncation 2.5 d^3 for a starting error d, far below rounding): five operations where two
+// Newton steps take eight, the same accuracy (max 1.4e-16 relative in a CPU emulation with
+// |d| <= 2^-22).
 __device__ __forceinline__ double rsqrt_f64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double d = __builtin_fma(-x * y, y, 1.0);
-  y = __builtin_fma(0.5 * y, d, y);
-  d = __builtin_fma(-x * y, y, 1.0);
-  y = __builtin_fma(0.5 * y, d, y);
-  return y;
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = __builtin_fma(-x * y, y, 1.0);
+  const double c = __builtin_fma(e, 0.375, 0.5);
+  return __builtin_fma(y * e, c, y);
 }
 
 // F_0..F_L at T.  T < 36: 8-term Taylor expansion of F_L about the nearest grid point
