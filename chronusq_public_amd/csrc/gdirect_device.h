@@ -75,9 +75,11 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 }
 
 // F_0..F_L at T.  T < 36: 8-term Taylor expansion of F_L about the nearest grid point
-// (|dT| <= 0.05, truncation below 1e-15 relative), exp(-T) = exp(-T0) exp(dT) by an
-// 8-term series, downward recursion F_m = (2T F_{m+1} + e^-T) / (2m+1); for L = 1 both F_0 and
-// F_1 by their own series (fewer operations than the exponential and the recursion).
+// (|dT| <= 0.05, truncation below 1e-15 relative), F(T0 - dT) = sum_k F_{L+k}(T0) dT^k / k!;
+// for L >= 2 exp(-T) = exp(-T0) exp(dT) by an 8-term series and the downward recursion
+// F_m = (2T F_{m+1} + e^-T) / (2m+1).  For L <= 1 the staged coefficients are already divided
+// by k! (one FMA per term), and for L = 1 the slice holds F_0..F_8 / k!: F_0 is the polynomial
+// and F_1 = -dF_0/dT its derivative in dT, both from one Horner pass (8 terms each).
 // T >= 36: F_0 = sqrt(pi/T)/2, upward recursion F_{m+1} = (2m+1)/(2T) F_m (the dropped
 // e^-T term is below 2.4e-16).
 template <int L>
@@ -89,35 +91,20 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ l
   const double Tt = big ? 0.0 : T;
   const int i = (int)(Tt * BOYS_INV_STEP + 0.5);
   const double dT = (double)i * BOYS_STEP - Tt;
-  const double* f0 = lds + i * BOYS_LDS_W;
-  const double* f = L == 1 ? f0 + 1 : f0;   // F_L .. F_{L+7}
-  double s = f[7];
-  s = f[6] + s * dT * (1.0 / 7.0);
-  s = f[5] + s * dT * (1.0 / 6.0);
-  s = f[4] + s * dT * (1.0 / 5.0);
-  s = f[3] + s * dT * (1.0 / 4.0);
-  s = f[2] + s * dT * (1.0 / 3.0);
-  s = f[1] + s * dT * (1.0 / 2.0);
-  s = f[0] + s * dT;
-  const double Ta = big ? T : BOYS_TMAX;
-  const double rs = rsqrt_f64(Ta);
-  double fa = 0.88622692545275801365 * rs;   // sqrt(pi)/2 T^-1/2
-  if constexpr (L == 0) {
-    F[0] = big ? fa : s;
-  } else if constexpr (L == 1) {
-    // the L = 1 slice holds F_0 .. F_8 (stage_boys): F_0 by its own 8-term series (its
-    // coefficients F_0..F_7, the same multipliers dT/k) instead of e^{-T} and the recursion
-    double s0 = f0[7];
-    s0 = f0[6] + s0 * dT * (1.0 / 7.0);
-    s0 = f0[5] + s0 * dT * (1.0 / 6.0);
-    s0 = f0[4] + s0 * dT * (1.0 / 5.0);
-    s0 = f0[3] + s0 * dT * (1.0 / 4.0);
-    s0 = f0[2] + s0 * dT * (1.0 / 3.0);
-    s0 = f0[1] + s0 * dT * (1.0 / 2.0);
-    s0 = f0[0] + s0 * dT;
-    F[0] = big ? fa : s0;
-    F[1] = big ? fa * (0.5 * rs * rs) : s;
-  } else {
+  const double* f = lds + i * BOYS_LDS_W;
+  if constexpr (L >= 2) {
+    // unscaled coefficients F_L..F_{L+7} (measured faster for these kernels than the scaled form)
+    double s = f[7];
+    s = f[6] + s * dT * (1.0 / 7.0);
+    s = f[5] + s * dT * (1.0 / 6.0);
+    s = f[4] + s * dT * (1.0 / 5.0);
+    s = f[3] + s * dT * (1.0 / 4.0);
+    s = f[2] + s * dT * (1.0 / 3.0);
+    s = f[1] + s * dT * (1.0 / 2.0);
+    s = f[0] + s * dT;
+    const double Ta = big ? T : BOYS_TMAX;
+    const double rs = rsqrt_f64(Ta);
+    double fa = 0.88622692545275801365 * rs;   // sqrt(pi)/2 T^-1/2
     // table: F_L by Taylor, then downward recursion with e^{-T} (Taylor about T_i)
     double e = 1.0 + dT * (1.0 / 8.0);
     e = 1.0 + dT * (1.0 / 7.0) * e;
@@ -127,7 +114,7 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ l
     e = 1.0 + dT * (1.0 / 3.0) * e;
     e = 1.0 + dT * (1.0 / 2.0) * e;
     e = 1.0 + dT * e;
-    e *= f0[8];
+    e *= f[8];
     const double T2 = 2.0 * Tt;
     double Ft[L + 1];
     Ft[L] = s;
@@ -140,6 +127,25 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ l
     for (int m = 0; m < L; ++m) {
       fa = fa * (double)(2 * m + 1) * hr;
       F[m + 1] = big ? fa : Ft[m + 1];
+    }
+  } else {
+    const double Ta = big ? T : BOYS_TMAX;
+    const double rs = rsqrt_f64(Ta);
+    const double fa = 0.88622692545275801365 * rs;   // sqrt(pi)/2 T^-1/2
+    if constexpr (L == 1) {
+      double p = f[8], dp = 0.0;
+#pragma unroll
+      for (int k = 7; k >= 0; --k) {
+        dp = __builtin_fma(dp, dT, p);
+        p = __builtin_fma(p, dT, f[k]);
+      }
+      F[0] = big ? fa : p;
+      F[1] = big ? fa * (0.5 * rs * rs) : dp;
+    } else {
+      double s = f[7];
+#pragma unroll
+      for (int k = 6; k >= 0; --k) s = __builtin_fma(s, dT, f[k]);
+      F[0] = big ? fa : s;
     }
   }
 }
@@ -154,12 +160,15 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ l
 }
 
 // Stage the Boys slice for total angular momentum L into LDS (whole workgroup): per grid point
-// F_L .. F_{L+7} and e^{-T_i}; for L = 1, F_0 .. F_8 (boys_eval<1> sums two series).
+// F_{L+c}, c = 0..7, and e^{-T_i}; divided by c! for L = 0, and F_c / c!, c = 0..8, for L = 1.
 __device__ __forceinline__ void stage_boys(const double* __restrict__ tab, int L, double* lds) {
+  constexpr double ifact[9] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040,
+                               1.0 / 40320};
   for (int k = threadIdx.x; k < BOYS_NGRID * BOYS_LDS_W; k += blockDim.x) {
     const int i = k / BOYS_LDS_W, c = k - i * BOYS_LDS_W;
-    lds[k] = L == 1 ? tab[i * (BOYS_MTAB + 1) + c]
-                    : (c < 8) ? tab[i * (BOYS_MTAB + 1) + L + c] : tab[i * (BOYS_MTAB + 1) + BOYS_MTAB];
+    lds[k] = L == 1 ? tab[i * (BOYS_MTAB + 1) + c] * ifact[c]
+             : L == 0 ? (c < 8 ? tab[i * (BOYS_MTAB + 1) + c] * ifact[c] : tab[i * (BOYS_MTAB + 1) + BOYS_MTAB])
+                      : (c < 8) ? tab[i * (BOYS_MTAB + 1) + L + c] : tab[i * (BOYS_MTAB + 1) + BOYS_MTAB];
   }
   __syncthreads();
 }
