@@ -126,9 +126,9 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
 }
 
 // Group kernels.  Work items are (task region, group of cpi chunks) pairs from an atomic queue
-// (cpi divides the class pair's chunk count); each workgroup spreads the item's (chunk, block of
-// 64 entries) units over its wavefronts -- every member's entries in one dense run, the member
-// wavefront-uniform, one quartet per lane -- then flushes its LDS accumulators to HBM once:
+// (cpi divides the class pair's chunk count); each workgroup spreads the item's (chunk, member,
+// block of 64 of that member's entries) units over its wavefronts -- the member wavefront-
+// uniform, one quartet per lane -- then flushes its LDS accumulators to HBM once:
 //  * ket-keyed (eri_grp, DIG 0 or 1): K columns of the shared c shell and of every member's d
 //    shell (DIG 1; with DIG 0 K goes to HBM atomics directly), J(c, d_m) of every member, and J of
 //    every union bra;
