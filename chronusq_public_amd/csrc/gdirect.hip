@@ -862,8 +862,11 @@ struct cq_ctx {
   std::vector<int> shell_mol;
   std::vector<int> cmg;               // npc x (nmol + 1): first screening group of each molecule
   DevBuf<int> d_pair_mol, d_cmg;
+  // class-pair kernels run on the contraction's own stream and up to kAux more: four streams
+  // in all, one per hardware queue of the process (GPU_MAX_HW_QUEUES defaults to 4; a fifth
+  // stream would share a queue and serialise behind another's kernels)
   int n_streams = 4;
-  static constexpr int kAux = 4;
+  static constexpr int kAux = 3;
   hipStream_t aux[kAux] = {};
   hipEvent_t ev_fork = nullptr, ev_join[kAux] = {};
   DevBuf<PrimPair> d_prims;
@@ -1687,8 +1690,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const CPMeta& m = c->cps[tb][k];
       c->last_table[k] = tb;
       if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
-      hipStream_t sk = c->class_stream[k] >= 0 ? c->aux[c->class_stream[k] % c->n_streams]
-                                               : c->aux[launched++ % c->n_streams];
+      const int si = c->class_stream[k] >= 0 ? c->class_stream[k] % c->n_streams : launched++ % c->n_streams;
+      hipStream_t sk = si == 0 ? s : c->aux[si - 1];
       // kernel: bra-keyed groups, ket-keyed with LDS K columns where they fit, else HBM K atomics
       const int kv = m.flip ? 2 : (var >= 1 && group_lds(ck, 1, c->lds_n, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
       const size_t lds_bytes = group_lds(ck, kv, c->lds_n, m.mmax, m.ub);
@@ -2127,7 +2130,7 @@ int cq_set_class_digestion(cq_ctx_t* c, int k, int mode) {
 int cq_set_class_stream(cq_ctx_t* c, int k, int stream) {
   return guarded(c, [&] {
     if (k < 0 || k >= kNumClassPairs) throw std::invalid_argument("class pair index out of range");
-    if (stream < -1 || stream >= cq_ctx::kAux) throw std::invalid_argument("stream index must be -1 .. 3");
+    if (stream < -1 || stream > cq_ctx::kAux) throw std::invalid_argument("stream index must be -1 .. 3");
     c->class_stream[k] = stream;
   });
 }
@@ -2246,11 +2249,11 @@ int cq_get_task_plan(const cq_ctx_t* c, long long* entries, int* max_batches) {
   return CQ_OK;
 }
 
-// Number of streams the class-pair kernels are spread over (1 serialises them, for
-// per-kernel profiling; default 4).
+// Number of streams the class-pair kernels are spread over, the contraction's own stream
+// first (1 serialises them on it, for per-kernel profiling; default 4).
 int cq_set_launch_streams(cq_ctx_t* c, int n) {
   return guarded(c, [&] {
-    if (n < 1 || n > cq_ctx::kAux) throw std::invalid_argument("stream count out of range");
+    if (n < 1 || n > cq_ctx::kAux + 1) throw std::invalid_argument("stream count out of range");
     c->n_streams = n;
   });
 }

@@ -166,13 +166,15 @@ int cq_set_class_digestion(cq_ctx_t* ctx, int k, int mode);
 int cq_set_class_chunk_group(cq_ctx_t* ctx, int k, int cpi);
 int cq_get_class_digestion(const cq_ctx_t* ctx, int k, int* mode);
 
-/* Auxiliary stream (0..3) the kernels of class pair k are launched on, -1 (default) for
- * round robin in launch order (highest angular momentum first).  For a tuner that balances
- * the streams by measured kernel times; no effect on results. */
+/* Stream (0..3) the kernels of class pair k are launched on: 0 is the stream the contraction
+ * was called on, 1..3 the context's auxiliary streams; -1 (default) for round robin in launch
+ * order (highest angular momentum first).  For a tuner that balances the streams by measured
+ * kernel times; no effect on results. */
 int cq_set_class_stream(cq_ctx_t* ctx, int k, int stream);
 
 /* Number of HIP streams (1..4, default 4) the class-pair kernels of one contraction are spread
- * over; 1 runs them one after another (for per-kernel timing). */
+ * over, the stream the contraction was called on first; 1 runs them one after another on it
+ * (for per-kernel timing).  Four streams use the process's four hardware queues one each. */
 int cq_set_launch_streams(cq_ctx_t* ctx, int n);
 
 /* Device memory for the screened task lists (no reference counterpart: the reference screens
