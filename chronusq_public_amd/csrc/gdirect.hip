@@ -826,7 +826,7 @@ struct cq_ctx {
   std::vector<unsigned long long> last_counts;
   // profiling: events around every class-pair kernel of every contraction
   bool profiling = false;
-  bool fused_screen = true;   // one screening launch for the class pairs screened once per call
+  int screen_mode = CQ_SCREEN_PER_STREAM;   // how the class pairs screened once per call are launched
   std::vector<int> launch_order;
   std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof_events, prof_screen_events;
   std::vector<double> prof_ms, prof_screen_ms;
@@ -1646,26 +1646,48 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       evs.push_back({q0, q1});
     }
   };
-  // every class pair screened once per call (one batch) in one launch on s, before the fork:
-  // thousands of one-group wavefronts of all class pairs fill the device together
+  // stream of each class pair's kernels (0: s, i > 0: aux[i - 1]): its explicit choice or
+  // round robin in launch order (highest angular momentum first: the latency-bound kernels
+  // start early) over the class pairs that have work
+  int stream_of[kNumClassPairs];
+  {
+    int launched = 0;
+    for (int kk = kNumClassPairs - 1; kk >= 0; --kk) {
+      const int k = c->launch_order[kk];
+      const CPMeta& m = c->cps[variant_of(k) >= 2 ? 1 : 0][k];
+      stream_of[k] = -1;
+      if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
+      stream_of[k] = c->class_stream[k] >= 0 ? c->class_stream[k] % c->n_streams : launched++ % c->n_streams;
+    }
+  }
+  auto stream_at = [&](int si) { return si == 0 ? s : c->aux[si - 1]; };
+  // class pairs screened once per call (one batch) share screening launches: with
+  // CQ_SCREEN_PER_STREAM one per stream, ahead of that stream's class kernels, so a stream
+  // starts its first (high angular momentum, lightly screened) kernels while the others still
+  // screen; with CQ_SCREEN_ONE one launch on s before the fork
   std::vector<char> fused(kNumClassPairs, 0);
-  if (c->fused_screen) {
+  auto fused_list = [&](int si) {
     ScreenList sl = screen_list_base();
     for (int k = 0; k < kNumClassPairs; ++k) {
+      if (stream_of[k] < 0 || (si >= 0 && stream_of[k] != si)) continue;
       const int tb = variant_of(k) >= 2 ? 1 : 0;
-      const CPMeta& m = c->cps[tb][k];
-      if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
       const std::vector<int>& bs = c->batch_start[tb][k];
       if (bs.size() != 2) continue;
       add_screen_entry(sl, k, tb, bs[0], bs[1] - bs[0], c->batch_slots[tb][k][0]);
       fused[k] = 1;
     }
-    launch_screen(sl, s, c->prof_screen_events[kNumClassPairs], true);
-  }
-  // 4. per class pair (on the auxiliary streams, highest angular momentum first: those are the
-  //    latency-bound ones): screening into its task regions, then ERI + J/K digestion, one
-  //    density slot per pass (the lists are screened once and kept for every slot when the class
-  //    pair is one batch); joined back into s before the finalisation
+    return sl;
+  };
+  if (c->screen_mode == CQ_SCREEN_ONE) launch_screen(fused_list(-1), s, c->prof_screen_events[kNumClassPairs], true);
+  HIPCHK(hipEventRecord(c->ev_fork, s));
+  for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
+  if (c->screen_mode == CQ_SCREEN_PER_STREAM)
+    for (int si = 0; si < c->n_streams; ++si)
+      launch_screen(fused_list(si), stream_at(si), c->prof_screen_events[kNumClassPairs], true);
+  // 4. per class pair, on its stream: screening into its task regions (unless screened above),
+  //    then ERI + J/K digestion, one density slot per pass (the lists are screened once and kept
+  //    for every slot when the class pair is one batch); joined back into s before the
+  //    finalisation
   for (int s0 = 0; s0 < nslot; ++s0) {
     JKArgs jk{};
     jk.ncart = c->ncart;
@@ -1679,9 +1701,6 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
     jk.do_j[0] = slots[s0].do_j;
     jk.do_k[0] = slots[s0].do_k;
     jk.nonsym = flags + s0;
-    HIPCHK(hipEventRecord(c->ev_fork, s));
-    for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
-    int launched = 0;
     for (int kk = kNumClassPairs - 1; kk >= 0; --kk) {
       const int k = c->launch_order[kk];
       const ClassKernels& ck = kClassKernels[k];
@@ -1689,9 +1708,8 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       const int tb = var >= 2 ? 1 : 0;
       const CPMeta& m = c->cps[tb][k];
       c->last_table[k] = tb;
-      if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
-      const int si = c->class_stream[k] >= 0 ? c->class_stream[k] % c->n_streams : launched++ % c->n_streams;
-      hipStream_t sk = si == 0 ? s : c->aux[si - 1];
+      if (stream_of[k] < 0) continue;
+      hipStream_t sk = stream_at(stream_of[k]);
       // kernel: bra-keyed groups, ket-keyed with LDS K columns where they fit, else HBM K atomics
       const int kv = m.flip ? 2 : (var >= 1 && group_lds(ck, 1, c->lds_n, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
       const size_t lds_bytes = group_lds(ck, kv, c->lds_n, m.mmax, m.ub);
@@ -1745,10 +1763,10 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
         }
       }
     }
-    for (int a = 0; a < cq_ctx::kAux; ++a) {
-      HIPCHK(hipEventRecord(c->ev_join[a], c->aux[a]));
-      HIPCHK(hipStreamWaitEvent(s, c->ev_join[a], 0));
-    }
+  }
+  for (int a = 0; a < cq_ctx::kAux; ++a) {
+    HIPCHK(hipEventRecord(c->ev_join[a], c->aux[a]));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_join[a], 0));
   }
   }
   // 5. back to the shells' basis and accumulate, every matrix in one launch:
@@ -2258,10 +2276,15 @@ int cq_set_launch_streams(cq_ctx_t* c, int n) {
   });
 }
 
-// One screening launch for every class pair that screens once per call (on), or one per
-// class pair on its own stream (off).  The surviving quartets are the same either way.
+// How the class pairs that screen once per call are screened: CQ_SCREEN_PER_CLASS (one launch
+// per class pair on its own stream), CQ_SCREEN_PER_STREAM (one launch per stream for the class
+// pairs on it; the default) or CQ_SCREEN_ONE (one launch for all, before the fork).  The
+// surviving quartets are the same in every mode.
 int cq_set_screen_fusion(cq_ctx_t* c, int on) {
-  return guarded(c, [&] { c->fused_screen = on != 0; });
+  return guarded(c, [&] {
+    if (on < CQ_SCREEN_PER_CLASS || on > CQ_SCREEN_ONE) throw std::invalid_argument("unknown screening mode");
+    c->screen_mode = on;
+  });
 }
 
 // Per-class-pair kernel timing with HIP events recorded on the launch stream.

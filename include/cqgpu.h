@@ -211,15 +211,20 @@ int cq_set_profiling(cq_ctx_t* ctx, int on);
 int cq_get_kernel_times(cq_ctx_t* ctx, double* ms, long long* launches, int max_n);
 /* The same for each class pair's screening kernel (the canonical quartet loop and the
  * density-weighted Schwarz test, aointegrals_contract.h:96-131); entry CQ_NUM_CLASS_PAIRS
- * (when max_n > CQ_NUM_CLASS_PAIRS) is the fused launch that screens every class pair held
- * in one batch. */
+ * (when max_n > CQ_NUM_CLASS_PAIRS) holds the shared launches that screen several class pairs
+ * (cq_set_screen_fusion). */
 int cq_get_screen_times(cq_ctx_t* ctx, double* ms, long long* launches, int max_n);
 
-/* Screening in one launch for all class pairs that screen once per call (on, the default) or
- * one launch per class pair on its own stream (off); the surviving quartets are identical.
- * No reference counterpart (the reference screens inside its quartet loop,
+/* How the class pairs that screen once per call are screened: one launch per class pair on
+ * its own stream (CQ_SCREEN_PER_CLASS), one launch per stream for the class pairs that run on
+ * it, ahead of their kernels (CQ_SCREEN_PER_STREAM, the default), or one launch for all class
+ * pairs before any class kernel starts (CQ_SCREEN_ONE).  The surviving quartets are identical
+ * in every mode.  No reference counterpart (the reference screens inside its quartet loop,
  * aointegrals_contract.h:96-131). */
-int cq_set_screen_fusion(cq_ctx_t* ctx, int on);
+#define CQ_SCREEN_PER_CLASS 0
+#define CQ_SCREEN_PER_STREAM 1
+#define CQ_SCREEN_ONE 2
+int cq_set_screen_fusion(cq_ctx_t* ctx, int mode);
 
 /* Work sums of the last contraction per class pair, 4 doubles each: sum K_ab*K_cd
  * (primitive quartets), sum K_ab, Cartesian->spherical transform FLOPs, output integrals.

@@ -351,10 +351,11 @@ def test_group_tiles_match_oracle(max_group, union_bound, lds):
 
 
 def test_screen_fusion_matches_per_class_screening():
-    """All class pairs screened in one launch (the default) or one launch per class pair on its
-    stream, with a budget that leaves some class pairs in several batches (those screen per
-    batch) and without: J and K against the oracle, identical surviving counts, for two
-    density slots (UHF-style) and in the mixed-precision rounds ((H2O)8 / cc-pVDZ)."""
+    """The class pairs screened in one launch per stream (the default), in one launch for all,
+    or one launch per class pair on its stream, with a budget that leaves some class pairs in
+    several batches (those screen per batch) and without: J and K against the oracle,
+    identical surviving counts, for two density slots (UHF-style) and in the mixed-precision
+    rounds ((H2O)8 / cc-pVDZ)."""
     from chronusq_public_amd import _lib
     L = _lib.load()
     mol = M.water_cluster(8, seed=11, box_ang=M.scaled_box(8))
@@ -367,7 +368,7 @@ def test_screen_fusion_matches_per_class_screening():
         if budget:
             ao.set_task_budget(budget)
             assert ao.task_plan()[1] > 1
-        for fusion in (1, 0):
+        for fusion in (1, 2, 0):
             print(f"[fusion] budget={budget} fusion={fusion}", flush=True)
             _lib.check(L.cq_set_screen_fusion(ao._h, fusion), ao._h)
             J, K, KY = _gpu_jk(ao, [X, X, Y], [COULOMB, EXCHANGE, EXCHANGE], [1.0, -0.5, -0.5], 1e-12)
@@ -377,7 +378,7 @@ def test_screen_fusion_matches_per_class_screening():
     ao.setPrecision("mixed", 1e-5)
     res = []
     try:
-        for fusion in (1, 0):
+        for fusion in (1, 2, 0):
             _lib.check(L.cq_set_screen_fusion(ao._h, fusion), ao._h)
             J, K = _gpu_jk(ao, [X, X], [COULOMB, EXCHANGE], [1.0, -0.5], 1e-12)
             res.append((J, K, ao.precision_counts()))
@@ -385,8 +386,12 @@ def test_screen_fusion_matches_per_class_screening():
                 assert np.abs(a - b).max() < 1e-9 + 1e-6 * np.abs(b).max()
     finally:
         ao.setPrecision("fp64")
-    assert res[0][2] == res[1][2] and res[0][2][1] > 0
-    assert np.abs(res[0][0] - res[1][0]).max() < 1e-12 and np.abs(res[0][1] - res[1][1]).max() < 1e-12
+    assert res[0][2][1] > 0
+    for r in res[1:]:
+        assert r[2] == res[0][2]
+        assert np.abs(r[0] - res[0][0]).max() < 1e-12 and np.abs(r[1] - res[0][1]).max() < 1e-12
+    _lib.check(L.cq_set_screen_fusion(ao._h, 1), ao._h)
+    assert L.cq_set_screen_fusion(ao._h, 3) != 0
 
 
 @pytest.mark.parametrize("cpi", [1, 2, 3, 36])
