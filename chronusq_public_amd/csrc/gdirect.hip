@@ -857,7 +857,14 @@ struct cq_ctx {
   std::vector<int> grp, cls_grp_off;
   DevBuf<int> d_grp, d_cls_grp_off;
   // several molecules in one context: shell pairs and quartets only within a molecule
-  int nmol = 1, lds_n = 0;            // lds_n: the largest molecule's Cartesian functions
+  int nmol = 1;
+  // LDS accumulator rows: a K column (ket-keyed) or K row (bra-keyed) holds one entry per
+  // Cartesian function of the swept pairs' shells, i.e. of the shells whose angular momentum
+  // belongs to the swept pair class {la, lb}; lds_n_set[pair class] is the largest molecule's
+  // count of such functions, and d_rowfn[pair class * ncart + loff + r] the Cartesian function
+  // of row r of the molecule whose functions start at loff
+  int lds_n_set[6] = {};
+  DevBuf<int> d_rowfn;
   size_t max_dyn_lds = 0;             // dynamic LDS bytes a segment kernel may use
   std::vector<int> shell_mol;
   std::vector<int> cmg;               // npc x (nmol + 1): first screening group of each molecule
@@ -919,11 +926,18 @@ int persistent_grid() {
   return g;
 }
 
+// Rows of a class pair's LDS K accumulators: the functions of the swept side's shells (the bra
+// pair class for the ket-keyed kernels, the ket pair class for the bra-keyed ones).
+int lds_rows(const cq_ctx* c, const ClassKernels& ck, int variant) {
+  return c->lds_n_set[variant == 2 ? ck.ki : ck.bi];
+}
+
 // Dynamic LDS bytes of a group kernel: ket-keyed with LDS K columns (variant 1), with HBM K
 // atomics (variant 0), or bra-keyed (variant 2), for groups of up to mmax members and unions of
 // up to ub swept pairs.
-size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int ub) {
+size_t group_lds(const cq_ctx* c, const ClassKernels& ck, int variant, int mmax, int ub) {
   const size_t ncd = (size_t)ck.nc * ck.nd, nab = (size_t)ck.na * ck.nb;
+  const size_t lds_n = (size_t)lds_rows(c, ck, variant);
   // (the last term: one int per union slot, its swept pair)
   if (variant == 2)
     return (((size_t)ck.na + (size_t)mmax * ck.nb) * lds_n + mmax * nab + ub * ncd) * sizeof(double) + ub * sizeof(int);
@@ -933,16 +947,20 @@ size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int u
 
 // Tile of a class pair for one digestion variant: the union bound (J slots of the swept side,
 // 64 .. 1024 pairs) and the largest group (at most the cap, default 1) that keeps the kernel's
-// dynamic LDS within the budget (default 48 KiB: with the 26 KiB Boys table two workgroups share
-// a CU).  mmax = 0: the variant does not fit the device even with single-pair groups.
+// dynamic LDS within the budget (default 48 KiB).  mmax = 0: the variant does not fit the
+// device even with single-pair groups.
 // Default union bound per class pair (kernel orientation), from a sweep of 256..1024 with
 // single-pair groups on (H2O)16/cc-pVDZ, tau = 1e-12 (per-class kernel + screening time): the
 // bound trades fewer K-column flushes (large) against more work items to balance (small).
 // Built-in tile of each class pair: the largest group (uniform pairs sharing a shell) and the
 // union bound (swept pairs per task region), the group size then lowered until the LDS budget
-// holds it.  One molecule (measured on (H2O)16/cc-pVDZ with screening in one launch and
-// per-member wavefront units): groups of 3-8 members pay where a swept pair's J slot would
-// otherwise be flushed to HBM once per uniform pair ((ps|ps) -10 %, (dp|pp) -41 %); elsewhere
+// holds it.  One molecule (measured on (H2O)16/cc-pVDZ with per-member wavefront units and
+// LDS rows for the swept shells only): groups of 4-16 members pay where a swept pair's J slot
+// would otherwise be flushed to HBM once per uniform pair ((ps|ps) kernel 0.82 -> 0.37 ms from
+// 1 to 14 members, (ss|ps) 0.95 -> 0.61 ms at 4; G[D] 4.23 -> 4.14 ms for the whole table,
+// although the screening launch's one wavefront per group then runs longer: 0.48 -> 0.62 ms
+// serialised, hidden behind the other streams' kernels); larger groups cost where a class
+// pair has few work items ((dp|pp) 0.21 ms with one member, 0.29 with 8); elsewhere
 // single-pair groups are as fast.  Several molecules in one context (measured on 256 alkanes and water clusters,
 // 6-31G*, groups of 4096 functions): a uniform pair has few partners inside its small
 // molecule, so large groups fill the wavefronts (serialised kernel time 306 -> 273 ms).
@@ -952,9 +970,9 @@ size_t group_lds(const ClassKernels& ck, int variant, int lds_n, int mmax, int u
 // (dp|dp) 0.29 -> 0.17 ms at 3, (dd|dd) 0.14 -> 0.08 ms; the batch: G[D] 152 -> 118 ms with all.
 struct TileDefault { const char* name; int max_group, ub, cpi; };
 static const TileDefault kTileDefaults[] = {
-    {"ssss", 4, 256, 1}, {"ssps", 4, 256, 1}, {"psps", 8, 256, 1}, {"sspp", 2, 512, 1}, {"ppps", 4, 512, 1},
-    {"pppp", 3, 256, 1}, {"ssds", 1, 512, 1}, {"psds", 3, 512, 1}, {"ppds", 1, 256, 1}, {"dsds", 1, 256, 1},
-    {"ssdp", 1, 512, 1}, {"dpps", 1, 256, 1}, {"dppp", 8, 256, 3}, {"dpds", 1, 256, 1}, {"dpdp", 1, 512, 3},
+    {"ssss", 4, 256, 1}, {"ssps", 4, 256, 1}, {"psps", 16, 256, 1}, {"sspp", 4, 512, 1}, {"ppps", 12, 512, 1},
+    {"pppp", 3, 256, 1}, {"ssds", 4, 256, 1}, {"psds", 8, 256, 1}, {"ppds", 1, 256, 1}, {"dsds", 1, 256, 1},
+    {"ssdp", 1, 512, 1}, {"dpps", 1, 256, 1}, {"dppp", 1, 256, 3}, {"dpds", 1, 256, 1}, {"dpdp", 1, 512, 3},
     {"ddss", 1, 256, 1}, {"ddps", 1, 512, 1}, {"ddpp", 1, 512, 1}, {"ddds", 1, 512, 1}, {"dddp", 1, 512, 2},
     {"dddd", 1, 256, 3}};
 static const TileDefault kTileDefaultsMulti[] = {
@@ -998,13 +1016,13 @@ void class_tile(const cq_ctx* c, int k, int variant, int& mmax, int& ub) {
   const size_t budget = c->tile_lds[k] > 0 ? (size_t)c->tile_lds[k] : (size_t)48 * 1024;
   const size_t limit = std::min(budget, c->max_dyn_lds);
   // a union bound whose J slots alone overflow the device's LDS is lowered (multiples of 64)
-  while (ub > 64 && group_lds(ck, variant, c->lds_n, 1, ub) > c->max_dyn_lds) ub -= 64;
+  while (ub > 64 && group_lds(c, ck, variant, 1, ub) > c->max_dyn_lds) ub -= 64;
   mmax = 0;
   const TileDefault* t = tile_default(c, ck);
   const int cap = c->tile_mmax[k] > 0 ? std::min(c->tile_mmax[k], MAX_GROUP) : (t ? t->max_group : 1);
   for (int mm = cap; mm >= 1; --mm)
-    if (group_lds(ck, variant, c->lds_n, mm, ub) <= limit) { mmax = mm; break; }
-  if (mmax == 0 && group_lds(ck, variant, c->lds_n, 1, ub) <= c->max_dyn_lds) mmax = 1;
+    if (group_lds(c, ck, variant, mm, ub) <= limit) { mmax = mm; break; }
+  if (mmax == 0 && group_lds(c, ck, variant, 1, ub) <= c->max_dyn_lds) mmax = 1;
 }
 
 // Class-pair metadata, uniform groups, task-list capacities and batches of groups.  Table 1 keys
@@ -1203,9 +1221,30 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->nmol = nshell > 0 ? c->shell_mol[nshell - 1] + 1 : 1;
   std::vector<int> mol_cart0(c->nmol + 1, cart);
   for (int s = nshell - 1; s >= 0; --s) mol_cart0[c->shell_mol[s]] = c->shells[s].cart_off;
-  c->lds_n = 0;
-  for (int m = 0; m < c->nmol; ++m) c->lds_n = std::max(c->lds_n, mol_cart0[m + 1] - mol_cart0[m]);
   c->npc = 6;   // ss ps pp ds dp dd
+  // LDS rows per pair class: shell s's first row among the functions of its molecule's shells
+  // with angular momentum in {la, lb} (crow, -1 outside the set), and the inverse map
+  std::vector<int> crow((size_t)c->npc * nshell, -1);
+  {
+    std::vector<int> rowfn((size_t)c->npc * std::max(cart, 1), 0);
+    for (int la = 0; la <= 2; ++la)
+      for (int lb = 0; lb <= la; ++lb) {
+        const int pc = pair_class(la, lb);
+        c->lds_n_set[pc] = 0;
+        int r = 0;
+        for (int sh = 0; sh < nshell; ++sh) {
+          if (sh > 0 && c->shell_mol[sh] != c->shell_mol[sh - 1]) r = 0;
+          if (l[sh] != la && l[sh] != lb) continue;
+          crow[(size_t)pc * nshell + sh] = r;
+          const int nf = (l[sh] + 1) * (l[sh] + 2) / 2;
+          for (int i = 0; i < nf; ++i)
+            rowfn[(size_t)pc * cart + mol_cart0[c->shell_mol[sh]] + r + i] = c->shells[sh].cart_off + i;
+          r += nf;
+          c->lds_n_set[pc] = std::max(c->lds_n_set[pc], r);
+        }
+      }
+    c->d_rowfn.upload(rowfn, c->stream);
+  }
   // solid-harmonic tables for l = 0..2
   c->tco_off.assign(5, 0);
   for (int L = 0; L <= 4; ++L) {
@@ -1230,6 +1269,8 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
       h.info.a0 = c->shells[a].cart_off; h.info.b0 = c->shells[b].cart_off;
       h.info.loff = mol_cart0[c->shell_mol[s1]];
       h.cls = pair_class(l[a], l[b]);
+      h.info.ar = crow[(size_t)h.cls * nshell + a];
+      h.info.br = crow[(size_t)h.cls * nshell + b];
       const double* A = origin + 3 * a;
       const double* B = origin + 3 * b;
       const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
@@ -1691,7 +1732,6 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   for (int s0 = 0; s0 < nslot; ++s0) {
     JKArgs jk{};
     jk.ncart = c->ncart;
-    jk.lds_n = c->lds_n;
     double* base = work + (size_t)s0 * 5 * nc2;
     jk.D[0] = base;
     jk.GJ[0] = base + nc2;
@@ -1711,12 +1751,15 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
       if (stream_of[k] < 0) continue;
       hipStream_t sk = stream_at(stream_of[k]);
       // kernel: bra-keyed groups, ket-keyed with LDS K columns where they fit, else HBM K atomics
-      const int kv = m.flip ? 2 : (var >= 1 && group_lds(ck, 1, c->lds_n, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
-      const size_t lds_bytes = group_lds(ck, kv, c->lds_n, m.mmax, m.ub);
+      const int kv = m.flip ? 2 : (var >= 1 && group_lds(c, ck, 1, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
+      const size_t lds_bytes = group_lds(c, ck, kv, m.mmax, m.ub);
       if (m.mmax < 1 || lds_bytes > c->max_dyn_lds)
         throw std::runtime_error(std::string("class pair ") + ck.name + ": the planned tile does not fit the device's LDS");
       GroupKernel kern = ck.grp[fp32 ? 1 : 0][kv];
       const int wgs = kv == 2 ? WG : ck.seg_wg;
+      JKArgs jkk = jk;   // the LDS rows of this class pair's swept side
+      jkk.lds_n = lds_rows(c, ck, kv);
+      jkk.rowfn = c->d_rowfn.p + (size_t)(kv == 2 ? ck.ki : ck.bi) * c->ncart;
       const int cpi = class_chunks_per_item(c, k);
       unsigned* wq = c->d_work_q.p + ((size_t)s0 * kNumClassPairs + k) * WORK_QUEUES;
       // batches of groups, in order on this class pair's stream: a class pair in one batch
@@ -1754,7 +1797,7 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
                                                    m.seg_cap * (ck.nchunk / cpi));
         hipLaunchKernelGGL(kern, dim3(std::max(1, sgrid)), dim3(wgs), lds_bytes, sk, c->d_tasks.p + m.task_off,
                            c->d_segs.p + m.seg_off, nslots, c->d_ovf.p + k, wq, c->d_pairs.p, c->d_prims.p,
-                           c->d_boys.p, jk,
+                           c->d_boys.p, jkk,
                            c->d_groups[tb].p, c->d_uorder.p, m.mmax, m.ub, cpi);
         HIPCHK(hipGetLastError());
         if (profiling) {
@@ -2250,8 +2293,8 @@ int cq_get_class_tile(const cq_ctx_t* c, int k, int variant, int* max_group, int
   *max_group = m.mmax;
   *union_bound = m.ub;
   *ngroups = m.ngrp;
-  const int kv = m.flip ? 2 : (variant >= 1 && group_lds(ck, 1, c->lds_n, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
-  *lds_bytes = (long long)group_lds(ck, kv, c->lds_n, m.mmax, m.ub);
+  const int kv = m.flip ? 2 : (variant >= 1 && group_lds(c, ck, 1, m.mmax, m.ub) <= c->max_dyn_lds) ? 1 : 0;
+  *lds_bytes = (long long)group_lds(c, ck, kv, m.mmax, m.ub);
   return CQ_OK;
 }
 

@@ -35,6 +35,8 @@ struct PairInfo {
   int p0, p1;      // primitive-pair range in the PrimPair array
   int a0, b0;      // first Cartesian function of a and b
   int loff;        // first Cartesian function of the pair's molecule (0 for one molecule)
+  int ar, br;      // first LDS row of a and b: their place among the functions of the
+                   // molecule's shells whose angular momentum is la or lb (the pair's class)
 };
 
 struct QuartetArgs {
@@ -44,14 +46,16 @@ struct QuartetArgs {
   double deg;
   bool ket_uniform;          // every lane of the wavefront holds the same ket pair
   bool bra_swap, ket_swap;   // pair held reversed relative to its canonical (s1 >= s2) order
-  int loff;                  // LDS row origin: first Cartesian function of the molecule
+  int ar, br, cr, dr;        // LDS rows of the swept pair's shells (PairInfo::ar, br)
 };
 
 // One density slot per kernel launch (a call with several densities runs one pass per slot).
 #define MAX_SLOTS 1
 struct JKArgs {
   int ncart;
-  int lds_n;                     // rows of an LDS accumulator column: the largest molecule's ncart
+  int lds_n;                     // rows of an LDS accumulator column (or row): the largest
+                                 // molecule's functions of the swept pair class's shells
+  const int* rowfn;              // Cartesian function of LDS row r: rowfn[loff + r]
   const double* D[MAX_SLOTS];    // density in the Cartesian basis, row-major ncart x ncart
   double* GJ[MAX_SLOTS];         // forward Coulomb updates
   double* GJ2[MAX_SLOTS];        // forward Coulomb updates from the transposed density

@@ -64,7 +64,8 @@ __device__ __forceinline__ void make_quartet_uniform_ket(const PairInfo& B, unsi
   q.ket_uniform = true;
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
-  q.loff = K.loff;   // the uniform ket's molecule (every quartet of a group lies in it)
+  q.ar = B.ar; q.br = B.br;   // LDS rows of the swept bra's shells
+  q.cr = 0; q.dr = 0;
 }
 
 // Quartet of the wavefront-uniform bra pair bb and its primitive block bblk (from
@@ -89,7 +90,8 @@ __device__ __forceinline__ void make_quartet_uniform_bra(const PairInfo& K, unsi
   q.ket_uniform = false;
   q.bra_swap = B.a != B.s1;
   q.ket_swap = K.a != K.s1;
-  q.loff = B.loff;   // the uniform bra's molecule
+  q.ar = 0; q.br = 0;
+  q.cr = K.ar; q.dr = K.br;   // LDS rows of the swept ket's shells
 }
 
 // Take the next work item (whole workgroup).  Up to eight queues, one per XCD (workgroups are
@@ -206,7 +208,7 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
         for (int k = threadIdx.x; k < (NC + n * ND) * LN; k += WGS) {                          \
           const double v = colK[k];                                                           \
           if (v != 0.0) {                                                                     \
-            const int col = k / LN, row = k - col * LN + K0.loff;                             \
+            const int col = k / LN, row = jk.rowfn[K0.loff + k - col * LN];                   \
             const int md = (col - NC) / ND;                                                   \
             const int gcol = col < NC ? K0.a0 + col : s_dcol[md] + (col - NC - md * ND);       \
             atomicAdd(&jk.GK[0][(long)row * N + gcol], v);                                    \
@@ -325,7 +327,7 @@ __device__ __forceinline__ long jcode_addr(int code, int N, int i, int j) {
       for (int k = threadIdx.x; k < (NA + n * NB) * LN; k += WG) {                              \
         const double v = rowK[k];                                                               \
         if (v != 0.0) {                                                                         \
-          const int lrow = k / LN, col = k - lrow * LN + A0.loff;                               \
+          const int lrow = k / LN, col = jk.rowfn[A0.loff + k - lrow * LN];                     \
           const int mb = (lrow - NA) / NB;                                                      \
           const int grow = lrow < NA ? A0.a0 + lrow : s_brow[mb] + (lrow - NA - mb * NB);        \
           atomicAdd(&jk.GK[0][(long)grow * N + col], v);                                        \

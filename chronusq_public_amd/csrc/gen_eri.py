@@ -367,7 +367,7 @@ def _jk_addressing(em: Emitter):
       tab: (1,2) = (s1,s2) of the bra   xba: (2,1)
       tcd: (3,4) = (s3,s4) of the ket   xdc: (4,3)"""
     em("const int N = jk.ncart;")
-    em("const int LN = jk.lds_n;   // LDS accumulator rows: the largest molecule's functions")
+    em("const int LN = jk.lds_n;   // LDS accumulator rows: the swept shells' functions")
     em("const int tab = q.bra_swap ? q.b0 * N + q.a0 : q.a0 * N + q.b0;")
     em("const int tab_a = q.bra_swap ? 1 : N, tab_b = q.bra_swap ? N : 1;")
     em("const int xba = q.bra_swap ? q.a0 * N + q.b0 : q.b0 * N + q.a0;")
@@ -460,9 +460,9 @@ def gen_digest_chunk(em: Emitter, NA, NB, NC, ND, ks, persist):
         if trans:
             return f"if (live) {glob};"
         colp = "colC" if colkind == "c" else "colD"
-        lds = f"atomicAdd(&{colp}[{ci} * LN + q.{row0} - q.loff + {ri}], double({val}))"
+        lds = f"atomicAdd(&{colp}[{ci} * LN + q.{row0[0]}r + {ri}], double({val}))"
         rowp = "colC" if row0 == "a0" else "colD"
-        ldsb = f"atomicAdd(&{rowp}[{ri} * LN + {col0} - q.loff + {ci}], double({val}))"
+        ldsb = f"atomicAdd(&{rowp}[{ri} * LN + {col0[:3]}r + {ci}], double({val}))"
         return (f"if (live) {{ if constexpr (DIG == 1) {lds}; else if constexpr (DIG == 2) {ldsb}; "
                 f"else {glob}; }}")
 
@@ -538,7 +538,7 @@ def gen_digest_final(em: Emitter, NA, NB, NC, ND):
     em("for (int s = 0; s < NSLOT; ++s) {")
     em.ind += 1
     em("const int N = jk.ncart;")
-    em("const int LN = jk.lds_n;   // LDS accumulator rows: the largest molecule's functions")
+    em("const int LN = jk.lds_n;   // LDS accumulator rows: the swept shells' functions")
     em("const int tab = q.bra_swap ? q.b0 * N + q.a0 : q.a0 * N + q.b0;")
     em("const int tab_a = q.bra_swap ? 1 : N, tab_b = q.bra_swap ? N : 1;")
     em("const R dg = R(q.deg);")
@@ -552,13 +552,13 @@ def gen_digest_final(em: Emitter, NA, NB, NC, ND):
     em("if (q.deg != 0.0 && jk.do_k[s]) {")
     for ia in range(NA):
         for idd in range(ND):
-            em(f"  if constexpr (DIG == 1) atomicAdd(&colD[{idd} * LN + q.a0 - q.loff + {ia}], double(hk * kad_{ia}_{idd}[s]));")
-            em(f"  else if constexpr (DIG == 2) atomicAdd(&colC[{ia} * LN + q.d0 - q.loff + {idd}], double(hk * kad_{ia}_{idd}[s]));")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colD[{idd} * LN + q.ar + {ia}], double(hk * kad_{ia}_{idd}[s]));")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colC[{ia} * LN + q.dr + {idd}], double(hk * kad_{ia}_{idd}[s]));")
             em(f"  else atomicAdd(&jk.GK[s][(q.a0 + {ia}) * N + q.d0 + {idd}], double(hk * kad_{ia}_{idd}[s]));")
     for ib in range(NB):
         for idd in range(ND):
-            em(f"  if constexpr (DIG == 1) atomicAdd(&colD[{idd} * LN + q.b0 - q.loff + {ib}], double(hk * kbd_{ib}_{idd}[s]));")
-            em(f"  else if constexpr (DIG == 2) atomicAdd(&colD[{ib} * LN + q.d0 - q.loff + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
+            em(f"  if constexpr (DIG == 1) atomicAdd(&colD[{idd} * LN + q.br + {ib}], double(hk * kbd_{ib}_{idd}[s]));")
+            em(f"  else if constexpr (DIG == 2) atomicAdd(&colD[{ib} * LN + q.dr + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
             em(f"  else atomicAdd(&jk.GK[s][(q.b0 + {ib}) * N + q.d0 + {idd}], double(hk * kbd_{ib}_{idd}[s]));")
     em("}")
     em.ind -= 1
