@@ -350,10 +350,10 @@ def main():
     ms = (ctypes.c_double * ncp)()
     nl = (ctypes.c_longlong * ncp)()
     _lib.check(L.cq_get_kernel_times(ao._h, ms, nl, ncp), ao._h)
-    sms = (ctypes.c_double * (ncp + 1))()      # entry ncp: the fused screening launch
+    sms = (ctypes.c_double * (ncp + 1))()      # entry ncp: the launches shared by several class pairs
     snl = (ctypes.c_longlong * (ncp + 1))()
     _lib.check(L.cq_get_screen_times(ao._h, sms, snl, ncp + 1), ao._h)
-    fused_screen_ms = sms[ncp] / snl[ncp] if snl[ncp] else 0.0
+    fused_screen_ms = sms[ncp] / args.steps     # per G[D] (two launches per stream and call)
     _lib.check(L.cq_set_profiling(ao._h, 0), ao._h)
     _lib.check(L.cq_set_launch_streams(ao._h, 4), ao._h)
     counts = ao.quartet_counts()

@@ -1703,28 +1703,40 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   }
   auto stream_at = [&](int si) { return si == 0 ? s : c->aux[si - 1]; };
   // class pairs screened once per call (one batch) share screening launches: with
-  // CQ_SCREEN_PER_STREAM one per stream, ahead of that stream's class kernels, so a stream
-  // starts its first (high angular momentum, lightly screened) kernels while the others still
-  // screen; with CQ_SCREEN_ONE one launch on s before the fork
+  // CQ_SCREEN_PER_STREAM each stream screens its first such class pair alone (high angular
+  // momentum: few quartets, a short launch), runs that pair's kernel, and screens the rest of
+  // its class pairs behind it, so the kernels start early and the bulk of the screening runs
+  // beside other streams' kernels; with CQ_SCREEN_ONE one launch on s before the fork screens
+  // them all
   std::vector<char> fused(kNumClassPairs, 0);
-  auto fused_list = [&](int si) {
-    ScreenList sl = screen_list_base();
-    for (int k = 0; k < kNumClassPairs; ++k) {
-      if (stream_of[k] < 0 || (si >= 0 && stream_of[k] != si)) continue;
-      const int tb = variant_of(k) >= 2 ? 1 : 0;
-      const std::vector<int>& bs = c->batch_start[tb][k];
-      if (bs.size() != 2) continue;
-      add_screen_entry(sl, k, tb, bs[0], bs[1] - bs[0], c->batch_slots[tb][k][0]);
+  std::vector<int> once[cq_ctx::kAux + 1];   // per stream, in launch order
+  if (c->screen_mode != CQ_SCREEN_PER_CLASS)
+    for (int kk = kNumClassPairs - 1; kk >= 0; --kk) {
+      const int k = c->launch_order[kk];
+      if (stream_of[k] < 0 || c->batch_start[variant_of(k) >= 2 ? 1 : 0][k].size() != 2) continue;
+      once[stream_of[k]].push_back(k);
       fused[k] = 1;
+    }
+  auto screen_of = [&](const int* ks, int n) {
+    ScreenList sl = screen_list_base();
+    for (int i = 0; i < n; ++i) {
+      const int k = ks[i], tb = variant_of(k) >= 2 ? 1 : 0;
+      const std::vector<int>& bs = c->batch_start[tb][k];
+      add_screen_entry(sl, k, tb, bs[0], bs[1] - bs[0], c->batch_slots[tb][k][0]);
     }
     return sl;
   };
-  if (c->screen_mode == CQ_SCREEN_ONE) launch_screen(fused_list(-1), s, c->prof_screen_events[kNumClassPairs], true);
+  if (c->screen_mode == CQ_SCREEN_ONE) {
+    std::vector<int> all;
+    for (int si = 0; si < c->n_streams; ++si) all.insert(all.end(), once[si].begin(), once[si].end());
+    launch_screen(screen_of(all.data(), (int)all.size()), s, c->prof_screen_events[kNumClassPairs], true);
+  }
   HIPCHK(hipEventRecord(c->ev_fork, s));
   for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
   if (c->screen_mode == CQ_SCREEN_PER_STREAM)
     for (int si = 0; si < c->n_streams; ++si)
-      launch_screen(fused_list(si), stream_at(si), c->prof_screen_events[kNumClassPairs], true);
+      if (!once[si].empty())
+        launch_screen(screen_of(once[si].data(), 1), stream_at(si), c->prof_screen_events[kNumClassPairs], true);
   // 4. per class pair, on its stream: screening into its task regions (unless screened above),
   //    then ERI + J/K digestion, one density slot per pass (the lists are screened once and kept
   //    for every slot when the class pair is one batch); joined back into s before the
@@ -1805,6 +1817,11 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
           c->prof_events[k].push_back({e0, e1});
         }
       }
+      // behind the first class pair of its stream, the screening of the stream's other
+      // once-per-call class pairs
+      const std::vector<int>& ol = once[stream_of[k]];
+      if (s0 == 0 && c->screen_mode == CQ_SCREEN_PER_STREAM && ol.size() > 1 && ol[0] == k)
+        launch_screen(screen_of(ol.data() + 1, (int)ol.size() - 1), sk, c->prof_screen_events[kNumClassPairs], true);
     }
   }
   for (int a = 0; a < cq_ctx::kAux; ++a) {
