@@ -1689,16 +1689,30 @@ void contract_device(cq_ctx* c, int nmat, const double* const* X, double* const*
   };
   // stream of each class pair's kernels (0: s, i > 0: aux[i - 1]): its explicit choice or
   // round robin in launch order (highest angular momentum first: the latency-bound kernels
-  // start early) over the class pairs that have work
+  // start early) over the class pairs that have work; the last, incomplete round runs from the
+  // last stream down, so its (lowest angular momentum, longest) kernels do not all land on the
+  // streams that already hold one kernel more ((H2O)16/cc-pVDZ: 21 class pairs, 4.11 -> 4.03 ms)
   int stream_of[kNumClassPairs];
   {
+    int nwork = 0;
+    for (int k = 0; k < kNumClassPairs; ++k) {
+      const CPMeta& m = c->cps[variant_of(k) >= 2 ? 1 : 0][k];
+      stream_of[k] = -1;
+      if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
+      if (c->class_stream[k] < 0) ++nwork;
+    }
+    const int full = nwork - nwork % c->n_streams;   // kernels in complete rounds
     int launched = 0;
     for (int kk = kNumClassPairs - 1; kk >= 0; --kk) {
       const int k = c->launch_order[kk];
       const CPMeta& m = c->cps[variant_of(k) >= 2 ? 1 : 0][k];
-      stream_of[k] = -1;
       if (m.nbra == 0 || m.nket == 0 || m.ngrp == 0) continue;
-      stream_of[k] = c->class_stream[k] >= 0 ? c->class_stream[k] % c->n_streams : launched++ % c->n_streams;
+      if (c->class_stream[k] >= 0) {
+        stream_of[k] = c->class_stream[k] % c->n_streams;
+      } else {
+        const int i = launched++;
+        stream_of[k] = i < full ? i % c->n_streams : c->n_streams - 1 - (i - full);
+      }
     }
   }
   auto stream_at = [&](int si) { return si == 0 ? s : c->aux[si - 1]; };

@@ -168,8 +168,9 @@ int cq_get_class_digestion(const cq_ctx_t* ctx, int k, int* mode);
 
 /* Stream (0..3) the kernels of class pair k are launched on: 0 is the stream the contraction
  * was called on, 1..3 the context's auxiliary streams; -1 (default) for round robin in launch
- * order (highest angular momentum first).  For a tuner that balances the streams by measured
- * kernel times; no effect on results. */
+ * order (highest angular momentum first; the last, incomplete round from the last stream
+ * down).  For a tuner that balances the streams by measured kernel times; no effect on
+ * results. */
 int cq_set_class_stream(cq_ctx_t* ctx, int k, int stream);
 
 /* Number of HIP streams (1..4, default 4) the class-pair kernels of one contraction are spread
