@@ -137,9 +137,11 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ l
     const double rs = rsqrt_f64(Ta);
     const double fa = 0.88622692545275801365 * rs;   // sqrt(pi)/2 T^-1/2
     if constexpr (L == 1) {
-      double p = f[8], dp = 0.0;
+      // (the first step of the derivative's recurrence, dp = 0 dT + f[8], taken out: the
+      // compiler cannot fold the product with zero itself)
+      double dp = f[8], p = __builtin_fma(f[8], dT, f[7]);
 #pragma unroll
-      for (int k = 7; k >= 0; --k) {
+      for (int k = 6; k >= 0; --k) {
         dp = __builtin_fma(dp, dT, p);
         p = __builtin_fma(p, dT, f[k]);
       }
