@@ -123,3 +123,46 @@ def test_batch_per_molecule_calls_match_merged():
     assert len(a.groups) > 1
     assert ra.converged.all() and rb.converged.all()
     assert np.abs(ra.energies - rb.energies).max() < 1e-9
+
+
+def test_batch_context_molecules_without_p_or_d_shells():
+    """The LDS accumulators of a batch context hold, per class pair, one row per function of
+    the swept pair class's shells in the largest molecule: molecules with no p or no d shells
+    at all (H2/6-31G*: s only; NH3/6-31G: s and p) beside one with s, p and d (H2O/cc-pVDZ),
+    the largest in the middle, give in each diagonal block the oracle's J and K (1e-10) with
+    the oracle's surviving-quartet counts."""
+    import torch
+    import oracle
+    from chronusq_public_amd import basis as B
+    from chronusq_public_amd.aointegrals import AOIntegrals, COULOMB, EXCHANGE
+    h2 = M.Molecule.from_angstrom(["H", "H"], [[0, 0, 0], [0, 0, 0.74]])
+    nh3 = M.Molecule.from_angstrom(["N", "H", "H", "H"],
+                                   [[0, 0, 0.12], [0.94, 0, -0.27], [-0.47, 0.81, -0.27], [-0.47, -0.81, -0.27]])
+    parts = [(h2, "6-31g*"), (M.water_test0001(), "cc-pvdz"), (nh3, "6-31g"), (h2, "6-31g*")]
+    bases = [B.build_basis(name, m.Z, m.coords_bohr) for m, name in parts]
+    assert max(b.max_l for b in bases[0::3]) == 0 and bases[2].max_l == 1 and bases[1].max_l == 2
+    big = AOIntegrals.batch(bases)
+    big.thresholdSchwartz = 1e-12
+    n = big.nBasis
+    rng = np.random.default_rng(5)
+    X = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    blocks, off = [], 0
+    for b in bases:
+        A = rng.uniform(-1, 1, (b.nbasis, b.nbasis))
+        A = np.asfortranarray(0.5 * (A + A.T))
+        blocks.append((off, b.nbasis, A))
+        X[off:off + b.nbasis, off:off + b.nbasis] = torch.from_numpy(A)
+        off += b.nbasis
+    J = torch.zeros_like(X)
+    K = torch.zeros_like(X)
+    big.newTwoEContractDirect([X, X], [J, K], [COULOMB, EXCHANGE], [1.0, -0.5])
+    nq = sum(big.quartet_counts().values())
+    total = 0
+    for (o, nb, A), b in zip(blocks, bases):
+        ob = oracle.OracleBasis(b)
+        Q = AOIntegrals(b).computeSchwartz()
+        AX = [np.zeros((nb, nb), order="F") for _ in range(2)]
+        total += ob.contract_direct(Q, 1e-12, [A, A], AX, [0, 1], [1.0, -0.5])
+        assert float(np.abs(J[o:o + nb, o:o + nb].cpu().numpy() - AX[0]).max()) < 1e-10
+        assert float(np.abs(K[o:o + nb, o:o + nb].cpu().numpy() - AX[1]).max()) < 1e-10
+    assert nq == total
