@@ -36,6 +36,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <exception>
+#include <thread>
 
 #include "../../include/cqgpu.h"
 
@@ -1254,47 +1256,82 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
       c->tco.insert(c->tco.end(), M.begin(), M.end());
     }
   }
-  // shell pairs (canonical s1 >= s2) and their primitive pairs
-  struct HostPair { PairInfo info; int cls; std::vector<PrimPair> prim; };
+  // shell pairs (canonical s1 >= s2) and their primitive pairs, built by up to 16 host threads
+  // over ranges of s1 (equal shares of the canonical pairs) and joined in s1 order, so the
+  // list is the same as a serial build's; a pair's primitive pairs are allprim[off, off + n)
+  struct HostPair { PairInfo info; int cls; long off; int n; };
   std::vector<HostPair> hp;
-  hp.reserve((size_t)nshell * (nshell + 1) / 2);
+  std::vector<PrimPair> allprim;
   const double pi = 3.14159265358979323846;
-  for (int s1 = 0; s1 < nshell; ++s1)
-    for (int s2 = 0; s2 <= s1; ++s2) {
-      if (c->shell_mol[s2] != c->shell_mol[s1]) continue;   // pairs only within a molecule
-      int a = s1, b = s2;
-      if (l[s2] > l[s1]) std::swap(a, b);
-      HostPair h;
-      h.info.a = a; h.info.b = b; h.info.s1 = s1; h.info.s2 = s2;
-      h.info.a0 = c->shells[a].cart_off; h.info.b0 = c->shells[b].cart_off;
-      h.info.loff = mol_cart0[c->shell_mol[s1]];
-      h.cls = pair_class(l[a], l[b]);
-      h.info.ar = crow[(size_t)h.cls * nshell + a];
-      h.info.br = crow[(size_t)h.cls * nshell + b];
-      const double* A = origin + 3 * a;
-      const double* B = origin + 3 * b;
-      const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
-      for (int i = 0; i < nprim[a]; ++i)
-        for (int j = 0; j < nprim[b]; ++j) {
-          const double ea = alpha[poff[a] + i], eb = alpha[poff[b] + j];
-          const double p = ea + eb;
-          const double Kab = std::exp(-ea * eb / p * AB2) * coef[poff[a] + i] * coef[poff[b] + j];
-          // primitive-pair bound sqrt((pp|pp)) of the s-type part
-          const double qp = std::fabs(Kab) * std::sqrt(2.0 * std::pow(pi, 2.5) / (p * p * std::sqrt(2.0 * p)));
-          if (qp < c->prim_threshold) continue;
-          PrimPair pp;
-          pp.p = p;
-          pp.K = Kab / p;
-          for (int k = 0; k < 3; ++k) {
-            pp.P[k] = (ea * A[k] + eb * B[k]) / p;
-            pp.PA[k] = pp.P[k] - A[k];
-            pp.PB[k] = pp.P[k] - B[k];
+  const double q_num = 2.0 * std::pow(pi, 2.5);
+  auto build_range = [&](int s1_lo, int s1_hi, std::vector<HostPair>& out, std::vector<PrimPair>& prims) {
+    for (int s1 = s1_lo; s1 < s1_hi; ++s1)
+      for (int s2 = 0; s2 <= s1; ++s2) {
+        if (c->shell_mol[s2] != c->shell_mol[s1]) continue;   // pairs only within a molecule
+        int a = s1, b = s2;
+        if (l[s2] > l[s1]) std::swap(a, b);
+        HostPair h;
+        h.info.a = a; h.info.b = b; h.info.s1 = s1; h.info.s2 = s2;
+        h.info.a0 = c->shells[a].cart_off; h.info.b0 = c->shells[b].cart_off;
+        h.info.loff = mol_cart0[c->shell_mol[s1]];
+        h.cls = pair_class(l[a], l[b]);
+        h.info.ar = crow[(size_t)h.cls * nshell + a];
+        h.info.br = crow[(size_t)h.cls * nshell + b];
+        const double* A = origin + 3 * a;
+        const double* B = origin + 3 * b;
+        const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+        h.off = (long)prims.size();
+        for (int i = 0; i < nprim[a]; ++i)
+          for (int j = 0; j < nprim[b]; ++j) {
+            const double ea = alpha[poff[a] + i], eb = alpha[poff[b] + j];
+            const double p = ea + eb;
+            const double Kab = std::exp(-ea * eb / p * AB2) * coef[poff[a] + i] * coef[poff[b] + j];
+            // primitive-pair bound sqrt((pp|pp)) of the s-type part
+            const double qp = std::fabs(Kab) * std::sqrt(q_num / (p * p * std::sqrt(2.0 * p)));
+            if (qp < c->prim_threshold) continue;
+            PrimPair pp;
+            pp.p = p;
+            pp.K = Kab / p;
+            for (int k = 0; k < 3; ++k) {
+              pp.P[k] = (ea * A[k] + eb * B[k]) / p;
+              pp.PA[k] = pp.P[k] - A[k];
+              pp.PB[k] = pp.P[k] - B[k];
+            }
+            pp.o2p = 0.5 / p;
+            prims.push_back(pp);
           }
-          pp.o2p = 0.5 / p;
-          h.prim.push_back(pp);
-        }
-      hp.push_back(std::move(h));
+        h.n = (int)((long)prims.size() - h.off);
+        out.push_back(h);
+      }
+  };
+  {
+    const int nthr = nshell < 64 ? 1 : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<int> cut(nthr + 1, nshell);
+    cut[0] = 0;
+    for (int t = 1; t < nthr; ++t)   // s1 < cut[t] holds about t / nthr of the canonical pairs
+      cut[t] = std::max(cut[t - 1], std::min(nshell, (int)std::lround(nshell * std::sqrt((double)t / nthr))));
+    std::vector<std::vector<HostPair>> parts(nthr);
+    std::vector<std::vector<PrimPair>> pparts(nthr);
+    std::vector<std::exception_ptr> errs(nthr);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthr; ++t)
+      pool.emplace_back([&, t] {
+        try { build_range(cut[t], cut[t + 1], parts[t], pparts[t]); } catch (...) { errs[t] = std::current_exception(); }
+      });
+    try { build_range(cut[0], cut[1], parts[0], pparts[0]); } catch (...) { errs[0] = std::current_exception(); }
+    for (auto& th : pool) th.join();
+    for (auto& e : errs)
+      if (e) std::rethrow_exception(e);
+    size_t total = 0, ptotal = 0;
+    for (int t = 0; t < nthr; ++t) { total += parts[t].size(); ptotal += pparts[t].size(); }
+    hp.reserve(total);
+    allprim.reserve(ptotal);
+    for (int t = 0; t < nthr; ++t) {
+      const long base = (long)allprim.size();
+      allprim.insert(allprim.end(), pparts[t].begin(), pparts[t].end());
+      for (HostPair h : parts[t]) { h.off += base; hp.push_back(h); }
     }
+  }
   // the one-electron integrals cover every shell pair; the two-electron lists drop the pairs
   // without a primitive pair above the threshold (Q = 0: none of their quartets can pass)
   {
@@ -1311,21 +1348,25 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     c->d_pairs_1e.upload(sorted_all, c->stream);
     HIPCHK(hipStreamSynchronize(c->stream));
   }
-  hp.erase(std::remove_if(hp.begin(), hp.end(), [](const HostPair& h) { return h.prim.empty(); }), hp.end());
+  hp.erase(std::remove_if(hp.begin(), hp.end(), [](const HostPair& h) { return h.n == 0; }), hp.end());
   // group by class (stable)
   std::stable_sort(hp.begin(), hp.end(), [](const HostPair& x, const HostPair& y) { return x.cls < y.cls; });
   c->class_start.assign(c->npc + 1, 0);
   for (auto& h : hp) c->class_start[h.cls + 1]++;
   for (int k = 0; k < c->npc; ++k) c->class_start[k + 1] += c->class_start[k];
   auto flatten = [&]() {
-    c->pairs.clear();
-    c->prims.clear();
-    for (auto& h : hp) {
-      PairInfo pi_ = h.info;
-      pi_.p0 = (int)c->prims.size();
-      c->prims.insert(c->prims.end(), h.prim.begin(), h.prim.end());
-      pi_.p1 = (int)c->prims.size();
-      c->pairs.push_back(pi_);
+    c->pairs.resize(hp.size());
+    size_t np_ = 0;
+    for (const auto& h : hp) np_ += (size_t)h.n;
+    c->prims.resize(np_);
+    size_t at = 0;
+    for (size_t k = 0; k < hp.size(); ++k) {
+      PairInfo pi_ = hp[k].info;
+      pi_.p0 = (int)at;
+      std::copy(allprim.begin() + hp[k].off, allprim.begin() + hp[k].off + hp[k].n, c->prims.begin() + at);
+      at += (size_t)hp[k].n;
+      pi_.p1 = (int)at;
+      c->pairs[k] = pi_;
     }
   };
   flatten();
@@ -1424,14 +1465,14 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     std::stable_sort(order.begin() + c->class_start[X], order.begin() + c->class_start[X + 1], [&](int x, int y) {
       const int mx = c->shell_mol[hp[x].info.s1], my = c->shell_mol[hp[y].info.s1];
       if (mx != my) return mx < my;
-      const size_t nx = hp[x].prim.size(), ny = hp[y].prim.size();
+      const int nx = hp[x].n, ny = hp[y].n;
       if (nx != ny) return nx > ny;
       return c->pairQ[x] > c->pairQ[y];
     });
   std::vector<HostPair> hp2;
   std::vector<double> q2;
   hp2.reserve(npair);
-  for (int k : order) { hp2.push_back(std::move(hp[k])); q2.push_back(c->pairQ[k]); }
+  for (int k : order) { hp2.push_back(hp[k]); q2.push_back(c->pairQ[k]); }
   hp.swap(hp2);
   c->pairQ.swap(q2);
   flatten();
