@@ -855,6 +855,8 @@ struct cq_ctx {
   DevBuf<int> d_cart2shell, d_sph2shell, d_tco_off, d_flags;
   DevBuf<double> d_tco, d_boys, d_pairQ, d_Nmax, d_Npair, d_work, d_mats;
   DevBuf<PairInfo> d_pairs, d_pairs_1e;
+  std::vector<PairInfo> pairs_1e_h;   // host copies kept for the asynchronous uploads
+  std::vector<int> rowfn_h;
   DevBuf<int2> d_s12;
   std::vector<int> grp, cls_grp_off;
   DevBuf<int> d_grp, d_cls_grp_off;
@@ -1228,7 +1230,8 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   // with angular momentum in {la, lb} (crow, -1 outside the set), and the inverse map
   std::vector<int> crow((size_t)c->npc * nshell, -1);
   {
-    std::vector<int> rowfn((size_t)c->npc * std::max(cart, 1), 0);
+    std::vector<int>& rowfn = c->rowfn_h;
+    rowfn.assign((size_t)c->npc * std::max(cart, 1), 0);
     for (int la = 0; la <= 2; ++la)
       for (int lb = 0; lb <= la; ++lb) {
         const int pc = pair_class(la, lb);
@@ -1341,12 +1344,13 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
     std::vector<int> order(all.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cls_all[x] < cls_all[y]; });
-    std::vector<PairInfo> sorted_all;
+    std::vector<PairInfo>& sorted_all = c->pairs_1e_h;
+    sorted_all.clear();
+    sorted_all.reserve(all.size());
     c->class_start_1e.assign(c->npc + 1, 0);
     for (int k : order) { sorted_all.push_back(all[k]); c->class_start_1e[cls_all[k] + 1]++; }
     for (int k = 0; k < c->npc; ++k) c->class_start_1e[k + 1] += c->class_start_1e[k];
     c->d_pairs_1e.upload(sorted_all, c->stream);
-    HIPCHK(hipStreamSynchronize(c->stream));
   }
   hp.erase(std::remove_if(hp.begin(), hp.end(), [](const HostPair& h) { return h.n == 0; }), hp.end());
   // group by class (stable)
