@@ -1358,18 +1358,14 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->class_start.assign(c->npc + 1, 0);
   for (auto& h : hp) c->class_start[h.cls + 1]++;
   for (int k = 0; k < c->npc; ++k) c->class_start[k + 1] += c->class_start[k];
+  // the pair list in hp's order; every pair's primitive pairs stay where the build put them
+  // (allprim, uploaded once), the pair pointing at its range
   auto flatten = [&]() {
     c->pairs.resize(hp.size());
-    size_t np_ = 0;
-    for (const auto& h : hp) np_ += (size_t)h.n;
-    c->prims.resize(np_);
-    size_t at = 0;
     for (size_t k = 0; k < hp.size(); ++k) {
       PairInfo pi_ = hp[k].info;
-      pi_.p0 = (int)at;
-      std::copy(allprim.begin() + hp[k].off, allprim.begin() + hp[k].off + hp[k].n, c->prims.begin() + at);
-      at += (size_t)hp[k].n;
-      pi_.p1 = (int)at;
+      pi_.p0 = (int)hp[k].off;
+      pi_.p1 = (int)(hp[k].off + hp[k].n);
       c->pairs[k] = pi_;
     }
   };
@@ -1392,6 +1388,8 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->boys_host = boys_table();   // stays alive until the stream is synchronised
   c->d_boys.upload(c->boys_host, c->stream);
   c->d_pairs.upload(c->pairs, c->stream);
+  if (allprim.size() > (size_t)INT32_MAX) throw std::invalid_argument("too many primitive pairs");
+  c->prims.swap(allprim);
   c->d_prims.upload(c->prims, c->stream);
   c->d_pairQ.ensure(npair);
   // Schwarz bounds: diagonal quartets (P|P) per class on the device, the classes on the
@@ -1481,7 +1479,6 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   c->pairQ.swap(q2);
   flatten();
   c->d_pairs.upload(c->pairs, c->stream);
-  c->d_prims.upload(c->prims, c->stream);
   c->d_pairQ.upload(c->pairQ, c->stream);
   // primitive-count groups per class: grp[cls_grp_off[X] .. cls_grp_off[X+1]] are the group
   // starts (class-local), closed by the class size
