@@ -1980,10 +1980,20 @@ void one_electron(cq_ctx* c, int natom, const double* Z, const double* xyz, cons
   int max_prim = 1;
   for (size_t k = 0; k + 1 < c->prim_off_host.size(); ++k)
     max_prim = std::max(max_prim, c->prim_off_host[k + 1] - c->prim_off_host[k]);
-  for (int X = 0; X < c->npc; ++X)
+  // the pair classes' kernels (small grids, latency-bound) side by side on s and the context's
+  // auxiliary streams; every class adds into its own blocks
+  HIPCHK(hipEventRecord(c->ev_fork, s));
+  for (int a = 0; a < cq_ctx::kAux; ++a) HIPCHK(hipStreamWaitEvent(c->aux[a], c->ev_fork, 0));
+  for (int X = 0; X < c->npc; ++X) {
+    const int si = X % (cq_ctx::kAux + 1);
     launch_onee(X, c->d_pairs_1e.p, c->class_start_1e[X], c->class_start_1e[X + 1], sd, max_prim, dZ, dxyz, natom,
-                max_atoms, c->d_boys.p, c->ncart, Sc, Tc, Vc, s);
+                max_atoms, c->d_boys.p, c->ncart, Sc, Tc, Vc, si == 0 ? s : c->aux[si - 1]);
+  }
   HIPCHK(hipGetLastError());
+  for (int a = 0; a < cq_ctx::kAux; ++a) {
+    HIPCHK(hipEventRecord(c->ev_join[a], c->aux[a]));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_join[a], 0));
+  }
   double* outs[3] = {S, T, V};
   for (int k = 0; k < 3; ++k)
     hipLaunchKernelGGL(sph_transform_kernel, dim3(grid_for((long)c->nsph * c->nsph, WG)), dim3(WG), 0, s,

@@ -68,7 +68,8 @@ __device__ __forceinline__ double hermE(int i, int j, int t, double PA, double P
   return s;
 }
 
-#define ONEE_ATOMS 4
+// nuclei per thread of the nuclear-attraction kernels (more slices: more threads in flight)
+#define ONEE_ATOMS 2
 
 template <int LA, int LB, bool DO_V>
 __global__ void __launch_bounds__(WG) onee_kernel(const PairInfo* __restrict__ pairs, int p_begin, int p_end,
