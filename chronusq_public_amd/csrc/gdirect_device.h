@@ -66,6 +66,11 @@ struct JKArgs {
   const int* nonsym;             // device flags: density slot is not symmetric
 };
 
+// a * b + c rounded once, in the integral type of a kernel (FP64, or FP32 for the mixed
+// window's kernels)
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // 1/sqrt(x) for x > 0: hardware estimate y0 (about 2^-23 relative), then one third-order
 // correction y = y0 (1 + e/2 + 3e^2/8), e = 1 - x y0^2 (the binomial series of (1 - e)^-1/2;
 // truncation 2.5 d^3 for a starting error d, far below rounding): five operations where two

@@ -308,11 +308,15 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                             cname = e
                         terms.append((tau, nu, ph, cname, sg))
             for (t, u, v) in herm_ab:
-                expr = ""
+                # one FMA into the accumulator per term (an expression summed first and then
+                # added would cost one more add per update)
+                w = f"W_{t}_{u}_{v}_{ic}_{idd}"
                 for (tau, nu, ph, cname, sg) in terms:
                     r = rg.get(0, t + tau, u + nu, v + ph)
-                    expr += f" {'+' if sg > 0 else '-'} " + (r if cname is None else f"{cname}*{r}")
-                em(f"W_{t}_{u}_{v}_{ic}_{idd} +={expr};")
+                    if cname is None:
+                        em(f"{w} {'+=' if sg > 0 else '-='} {r};")
+                    else:
+                        em(f"{w} = fmar({'' if sg > 0 else '-'}{cname}, {r}, {w});")
         em.ind -= 1
         em("}  // ket primitives")
         for ia, (ax, ay, az) in enumerate(ca):
@@ -332,8 +336,8 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
                             em(f"const R {bn} = bp_K{('*' + e) if e else ''};")
                             bterms.append((t, u, v, bn))
                 for (ic, idd) in ks:
-                    expr = " + ".join(f"{bn}*W_{t}_{u}_{v}_{ic}_{idd}" for (t, u, v, bn) in bterms)
-                    em(f"o_{ia}_{ib}_{ic}_{idd} += {expr};")
+                    for (t, u, v, bn) in bterms:
+                        em(f"o_{ia}_{ib}_{ic}_{idd} = fmar({bn}, W_{t}_{u}_{v}_{ic}_{idd}, o_{ia}_{ib}_{ic}_{idd});")
         em.ind -= 1
         em("}  // bra primitives")
         if kind == "block":
