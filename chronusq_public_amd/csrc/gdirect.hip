@@ -1266,7 +1266,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
   std::vector<HostPair> hp;
   std::vector<PrimPair> allprim;
   const double pi = 3.14159265358979323846;
-  const double q_num = 2.0 * std::pow(pi, 2.5);
+  const double q_num = 2.0 * std::pow(pi, 2.5), q_half = std::sqrt(q_num);
   auto build_range = [&](int s1_lo, int s1_hi, std::vector<HostPair>& out, std::vector<PrimPair>& prims) {
     for (int s1 = s1_lo; s1 < s1_hi; ++s1)
       for (int s2 = 0; s2 <= s1; ++s2) {
@@ -1294,7 +1294,7 @@ void build_context(cq_ctx* c, int nshell, const int* l, const int* pure, const i
             if (qp < c->prim_threshold) continue;
             PrimPair pp;
             pp.p = p;
-            pp.K = Kab / p;
+            pp.K = q_half * Kab / p;   // sqrt(2 pi^(5/2)) of the quartet prefactor per pair
             for (int k = 0; k < 3; ++k) {
               pp.P[k] = (ea * A[k] + eb * B[k]) / p;
               pp.PA[k] = pp.P[k] - A[k];

@@ -14,10 +14,10 @@
 #define BOYS_MTAB 24
 #define BOYS_LDS_W 9
 
-#define TWO_PI_52 34.986836655249725693   // 2 * pi^(5/2)
-
 // One primitive pair of a shell pair (a, b):
-//   p = alpha_a + alpha_b, K = exp(-alpha_a alpha_b / p |AB|^2) c_a c_b / p,
+//   p = alpha_a + alpha_b, K = sqrt(2 pi^(5/2)) exp(-alpha_a alpha_b / p |AB|^2) c_a c_b / p
+//   (a primitive quartet's prefactor 2 pi^(5/2) K_ab K_cd / (p q sqrt(p + q)) is then
+//   K_ab K_cd / sqrt(p + q), split evenly over its two pairs),
 //   P = (alpha_a A + alpha_b B) / p, PA = P - A, PB = P - B, o2p = 1 / (2p).
 struct alignas(16) PrimPair {
   double p;
@@ -97,7 +97,7 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ l
   // divergent branch (unrolled iterations then interleave): the table path at min(T, T_max)
   // (index always in range), the asymptotic path at max(T, T_max).
   const bool big = T >= BOYS_TMAX;
-  const double Tt = big ? 0.0 : T;
+  const double Tt = fmin(T, BOYS_TMAX);   // (the table's last point for the asymptotic range)
   const int i = (int)(Tt * BOYS_INV_STEP + 0.5);
   const double dT = (double)i * BOYS_STEP - Tt;
   const double* f = lds + i * BOYS_LDS_W;

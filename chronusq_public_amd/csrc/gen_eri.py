@@ -274,7 +274,7 @@ def gen_class(LA, LB, LC, LD, kind: str) -> str:
         em(f"boys_eval<{L}>(alphad * (PQxd * PQxd + PQyd * PQyd + PQzd * PQzd), boys_lds, F);")
         em("const R PQx = R(PQxd), PQy = R(PQyd), PQz = R(PQzd);")
         em("const R alpha = R(alphad);")
-        em("R pw = R(TWO_PI_52 * kp.K * rsd);")
+        em("R pw = R(kp.K * rsd);   // K of both pairs carry sqrt(2 pi^(5/2)) each")
         for n in range(L + 1):
             em(f"const R Rb{n} = F[{n}] * pw;")
             if n < L:
